@@ -1,0 +1,179 @@
+/*
+ * torchhull_amd: C ABI of the MI355X (gfx950) visual-hull library.
+ *
+ * Plain pointers and sizes only.  Every device pointer refers to memory of the
+ * context's device; `stream` is a hipStream_t (NULL = the default stream).
+ * Every entry point returns 0 on success and a nonzero status on failure; the
+ * message is then available from thull_last_error() (thread-local).  Calls on
+ * one context are not thread-safe: use one context per host thread.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (vc-bonn/torchhull, paths relative to the repository root).
+ */
+#ifndef TORCHHULL_AMD_H
+#define TORCHHULL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THULL_ABI_VERSION 1
+
+/* Element types (thull_dtype). */
+enum thull_dtype
+{
+    THULL_BOOL = 0,
+    THULL_U8 = 1,
+    THULL_I8 = 2,
+    THULL_I16 = 3,
+    THULL_I32 = 4,
+    THULL_I64 = 5,
+    THULL_F16 = 6,
+    THULL_F32 = 7,
+    THULL_F64 = 8
+};
+
+typedef struct thull_ctx thull_ctx;
+
+/* ABI version of the loaded library (THULL_ABI_VERSION). */
+int thull_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* thull_last_error(void);
+
+/* Context: a device plus pooled device memory reused across calls.
+ * Replaces the per-call torch caching-allocator / stdgpu allocations of the
+ * reference (src/torchhull/_C/include/torchhull/stdgpu_allocator.h:36-49). */
+int thull_ctx_create(int device, thull_ctx** out);
+int thull_ctx_destroy(thull_ctx* ctx);
+/* Peak pooled device bytes and currently reserved bytes. */
+int thull_ctx_memory(thull_ctx* ctx, int64_t* peak_bytes, int64_t* reserved_bytes);
+/* Release pooled memory that holds no result. */
+int thull_ctx_trim(thull_ctx* ctx);
+
+/* One visual-hull problem (the arguments of torchhull.visual_hull,
+ * src/torchhull/_C/python/bindings.cpp:12-21). */
+typedef struct thull_scene
+{
+    const void* masks;        /* device [B,H,W] contiguous; dtype THULL_F32, THULL_U8 or THULL_BOOL */
+    int32_t mask_dtype;
+    const float* transforms;  /* device [B,4,4] float32 row-major */
+    int64_t B, H, W;
+    int32_t level;            /* >= 0 */
+    int32_t masks_partial;    /* bool */
+    int32_t opengl;           /* 1: "opengl", 0: "opencv" */
+    float cube_corner_bfl[3];
+    float cube_length;        /* > 0 */
+} thull_scene;
+
+/* Per-scene result sizes of thull_visual_hull_run. */
+typedef struct thull_vh_counts
+{
+    int64_t num_verts;  /* mesh vertices (soup: 3 * num_faces) */
+    int64_t num_faces;
+    int64_t num_field;  /* sparse field entries (grid vertices of the leaves) */
+    int32_t num_levels; /* candidate lists kept (level + 1 unless carving emptied early) */
+    int32_t pad_;
+} thull_vh_counts;
+
+/* Carves and meshes `n_scenes` scenes in one batch: integral images, level-
+ * synchronous octree carving, corner vertices, field, sparse marching cubes
+ * at isolevel 0.5, world coordinates and face flip.  Replaces
+ * visual_hull_cuda (src/torchhull/_C/src/visual_hull.cpp:34-58) with
+ * sparse_visual_hull_field_cuda_ravelled (src/torchhull/_C/src/visual_hull_cuda.cu:582-894),
+ * marching_cubes_cuda_sparse (src/torchhull/_C/src/marching_cubes_cuda.cu:709-754) and
+ * to_global_coordinates_and_flip_faces_ (src/torchhull/_C/src/visual_hull_cuda.cu:958-1020).
+ * Results stay in the context until the next run; `counts` is host memory of
+ * n_scenes entries.  Blocks until the sizes are known. */
+int thull_visual_hull_run(thull_ctx* ctx,
+                          const thull_scene* scenes,
+                          int32_t n_scenes,
+                          int32_t unique_verts,
+                          void* stream,
+                          thull_vh_counts* counts);
+
+/* Copies scene `scene`'s mesh: verts float32 [num_verts,3], faces int64 [num_faces,3]. */
+int thull_visual_hull_fetch_mesh(thull_ctx* ctx, int32_t scene, float* verts, int64_t* faces, void* stream);
+
+/* Copies scene `scene`'s sparse field as COO: indices int64 [4,num_field] rows (0, z, y, x),
+ * values float32 [num_field]; replaces sparse_visual_hull_field_cuda
+ * (src/torchhull/_C/src/visual_hull_cuda.cu:896-914, to_sparse_coo_tensor :543-580). */
+int thull_visual_hull_fetch_field(thull_ctx* ctx, int32_t scene, int64_t* indices, float* values, void* stream);
+
+/* Number of candidates of single-scene runs at `level` (0..num_levels-1), and their
+ * ravelled ids (x + y*2^level + z*4^level) in carving order; replaces the candidate
+ * list of visual_hull_with_candidate_voxels (src/torchhull/_C/src/visual_hull.cpp:85-109). */
+int thull_visual_hull_candidates_count(thull_ctx* ctx, int32_t level, int64_t* count);
+int thull_visual_hull_fetch_candidates(thull_ctx* ctx, int32_t level, int64_t* out, void* stream);
+
+/* Camera tests (child-camera classifications, early exit included) and children
+ * tested at each carving step of the last run, summed over its scenes; host arrays of
+ * `max_levels` entries.  Instrumentation for throughput accounting. */
+int thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_tests, int64_t* children);
+
+/* Marching cubes on a dense volume [1,D,H,W] (value dtype THULL_F32 or THULL_F64)
+ * or a coalesced sparse COO volume (indices int64 [4,nnz], values [nnz]).
+ * return_local_coords maps to [-1,1]; otherwise grid units.  Replaces
+ * marching_cubes_cuda (src/torchhull/_C/src/marching_cubes_cuda.cu:817-831). */
+int thull_marching_cubes_dense_run(thull_ctx* ctx,
+                                   const void* volume,
+                                   int32_t dtype,
+                                   int64_t D,
+                                   int64_t H,
+                                   int64_t W,
+                                   float isolevel,
+                                   int32_t return_local_coords,
+                                   int32_t unique_verts,
+                                   void* stream,
+                                   int64_t* num_verts,
+                                   int64_t* num_faces);
+int thull_marching_cubes_sparse_run(thull_ctx* ctx,
+                                    const int64_t* indices,
+                                    const void* values,
+                                    int32_t dtype,
+                                    int64_t nnz,
+                                    int64_t D,
+                                    int64_t H,
+                                    int64_t W,
+                                    float isolevel,
+                                    int32_t return_local_coords,
+                                    int32_t unique_verts,
+                                    void* stream,
+                                    int64_t* num_verts,
+                                    int64_t* num_faces);
+int thull_marching_cubes_fetch(thull_ctx* ctx, float* verts, int64_t* faces, void* stream);
+
+/* Gaussian blur of images [B,H,W] (channel 0 of [B,H,W,1]) into out [B,H,W] of
+ * out_dtype (THULL_F16, THULL_F32 or THULL_F64); sparse = band blur.  Replaces
+ * gaussian_blur_cuda (src/torchhull/_C/src/gaussian_blur_cuda.cu:509-518). */
+int thull_gaussian_blur(thull_ctx* ctx,
+                        const void* images,
+                        int32_t in_dtype,
+                        int64_t B,
+                        int64_t H,
+                        int64_t W,
+                        int32_t kernel_size,
+                        float sigma,
+                        int32_t sparse,
+                        void* out,
+                        int32_t out_dtype,
+                        void* stream);
+
+/* float32 integral image of masks [B,H,W] (dtype THULL_F32/U8/BOOL); equal to the
+ * reference's integral_image (src/torchhull/_C/src/visual_hull_cuda.cu:76-90). */
+int thull_integral_image(thull_ctx* ctx,
+                         const void* masks,
+                         int32_t dtype,
+                         int64_t B,
+                         int64_t H,
+                         int64_t W,
+                         float* out,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHHULL_AMD_H */

@@ -1,0 +1,1014 @@
+// CPU restatement of the reference visual-hull pipeline (vc-bonn/torchhull).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is built into
+// oracle/_build/liboracle.so and is loaded only by tests/, by the smoke check
+// and by bench.py's CPU-baseline leg.  The product library never links it.
+//
+// Every function follows the reference step by step, with the same floating-
+// point operation order.  The file is compiled with -ffp-contract=off, so the
+// only fused multiply-adds are the explicit std::fma calls, placed exactly
+// where the reference calls fmaf or where nvcc contracts `a + b * c` by
+// default (--fmad=true).  Citations use the reference's paths relative to its
+// repository root; `VH` = src/torchhull/_C/src/visual_hull_cuda.cu,
+// `MC` = src/torchhull/_C/src/marching_cubes_cuda.cu,
+// `GB` = src/torchhull/_C/src/gaussian_blur_cuda.cu,
+// `IU` = src/torchhull/_C/include/torchhull/image_utils.h.
+//
+// Pinning status of each function is listed in oracle/README.md.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+
+#include "../torchhull_amd/csrc/mc_cases.h"
+
+namespace
+{
+
+constexpr thull::McCaseTable kCases = thull::mc_build_case_table();
+
+// ---------------------------------------------------------------------------
+// Scalar helpers
+// ---------------------------------------------------------------------------
+
+// float -> int64 with the semantics of CUDA's cvt.rzi.s64.f32 (what
+// static_cast<int64_t>(float) compiles to): truncate, saturate out-of-range
+// values and infinities, NaN -> 0.  Used at VH:239-240,358,368-369 and
+// IU:175-176.
+inline int64_t
+f2i64(float f)
+{
+    if (std::isnan(f))
+        return 0;
+    if (f >= 9223372036854775808.0f)
+        return std::numeric_limits<int64_t>::max();
+    if (f < -9223372036854775808.0f)
+        return std::numeric_limits<int64_t>::min();
+    return static_cast<int64_t>(f);
+}
+
+inline int64_t
+clamp64(int64_t v, int64_t lo, int64_t hi)
+{
+    return std::min(std::max(v, lo), hi);
+}
+
+// IU:56-60
+inline bool
+in_image(int64_t y, int64_t x, int64_t H, int64_t W, int64_t margin = 0)
+{
+    return margin <= y && y < H - margin && margin <= x && x < W - margin;
+}
+
+// IU:67-71: ((c + 1) * size) / 2 - 0.5, each step rounded
+inline float
+unnormalize_ndc(float c, int64_t size)
+{
+    float t = c + 1.0f;
+    t = t * static_cast<float>(size);
+    t = t / 2.0f;
+    return t - 0.5f;
+}
+
+// IU:78-82
+inline float
+align_cv(float c)
+{
+    return c - 0.5f;
+}
+
+// math.h:19-23
+inline float
+lerpf(float v0, float v1, float t)
+{
+    return std::fma(t, v1, std::fma(-t, v0, v0));
+}
+
+// math.h:25-32
+inline float
+bilerpf(float v00, float v10, float v01, float v11, float ty, float tx)
+{
+    float v0 = lerpf(v00, v10, ty);
+    float v1 = lerpf(v01, v11, ty);
+    return lerpf(v0, v1, tx);
+}
+
+// Projection of a world point to pixel coordinates.
+// VH:142-156 (row = fmaf(A0,x, fmaf(A1,y, fmaf(A2,z, A3)))), VH:217-228.
+inline void
+project(const float* T, float x, float y, float z, bool opengl, int64_t H, int64_t W, float& px, float& py)
+{
+    float cx = std::fma(T[0], x, std::fma(T[1], y, std::fma(T[2], z, T[3])));
+    float cy = std::fma(T[4], x, std::fma(T[5], y, std::fma(T[6], z, T[7])));
+    if (opengl)
+    {
+        float cw = std::fma(T[12], x, std::fma(T[13], y, std::fma(T[14], z, T[15])));
+        px = unnormalize_ndc(cx / cw, W);
+        py = unnormalize_ndc(cy / cw, H);
+    }
+    else
+    {
+        float cz = std::fma(T[8], x, std::fma(T[9], y, std::fma(T[10], z, T[11])));
+        px = align_cv(cx / cz);
+        py = align_cv(cy / cz);
+    }
+}
+
+// World coordinate of grid coordinate v at resolution res:
+// corner + v / res * length, which nvcc contracts to fmaf(v / res, length, corner)
+// (VH:206-213, VH:330-336, VH:444-448, VH:493-497).
+inline float
+grid_to_world(int64_t v, int64_t res, float length, float corner)
+{
+    return std::fma(static_cast<float>(v) / static_cast<float>(res), length, corner);
+}
+
+// ---------------------------------------------------------------------------
+// Integral image
+// ---------------------------------------------------------------------------
+
+// VH:76-90.  torch::cumsum(masks, 1, float32) then cumsum(.., 2, float32).  On
+// a [B,H,W,1] tensor both dimensions are outer (non-innermost) dimensions, so
+// ATen runs tensor_kernel_scan_outer_dim (ATen/native/cuda/ScanUtils.cuh),
+// which walks each line sequentially with a float accumulator starting at 0:
+// acc = acc + x.  The restatement does the same.
+void
+integral_image(const float* masks, int64_t B, int64_t H, int64_t W, float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < B; ++b)
+    {
+        const float* m = masks + b * H * W;
+        float* o = out + b * H * W;
+        // cumsum along H (dim 1), one sequential chain per column
+        for (int64_t x = 0; x < W; ++x)
+        {
+            float acc = 0.0f;
+            for (int64_t y = 0; y < H; ++y)
+            {
+                acc = acc + m[y * W + x];
+                o[y * W + x] = acc;
+            }
+        }
+        // cumsum along W (dim 2), one sequential chain per row
+        for (int64_t y = 0; y < H; ++y)
+        {
+            float acc = 0.0f;
+            for (int64_t x = 0; x < W; ++x)
+            {
+                acc = acc + o[y * W + x];
+                o[y * W + x] = acc;
+            }
+        }
+    }
+}
+
+inline float
+sample_zero(const float* img, int64_t H, int64_t W, int64_t y, int64_t x)
+{
+    return in_image(y, x, H, W) ? img[y * W + x] : 0.0f;
+}
+
+inline float
+sample_one(const float* img, int64_t H, int64_t W, int64_t y, int64_t x)
+{
+    return in_image(y, x, H, W) ? img[y * W + x] : 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Octree carving
+// ---------------------------------------------------------------------------
+
+struct Scene
+{
+    const float* masks;      // [B,H,W] float32 (masks of any dtype converted to float32, IU:178)
+    const float* integral;   // [B,H,W] float32
+    const float* transforms; // [B,16] float32 row-major
+    int64_t B, H, W;
+    bool opengl;
+    bool partial;
+    float corner[3];
+    float length;
+};
+
+// Classification of one child cell: VH:168-285 (full) and VH:287-417 (partial).
+// Returns the occupancy flag; k receives the number of cameras evaluated in
+// index order up to and including the first one that classifies the child as
+// empty (B when none does), the canonical early-exit count.
+bool
+classify_child(const Scene& s, int64_t cx, int64_t cy, int64_t cz, int64_t res_children, bool last_level, int64_t& k)
+{
+    bool is_empty = false;
+    bool should_refine = false;
+    bool fully_inside_one_frame = false;
+    const float margin = 0.5f - 1e-1f; // VH:260-267
+    k = s.B;
+    bool k_set = false;
+    for (int64_t b = 0; b < s.B; ++b)
+    {
+        const float* T = s.transforms + 16 * b;
+        float bb_min_x = FLT_MAX, bb_min_y = FLT_MAX;
+        float bb_max_x = -FLT_MAX, bb_max_y = -FLT_MAX;
+        bool fully_inside = true;
+        for (int i = 0; i < 8; ++i)
+        {
+            int64_t vx = cx + (i & 1), vy = cy + (i >> 1 & 1), vz = cz + (i >> 2 & 1);
+            float wx = grid_to_world(vx, res_children, s.length, s.corner[0]);
+            float wy = grid_to_world(vy, res_children, s.length, s.corner[1]);
+            float wz = grid_to_world(vz, res_children, s.length, s.corner[2]);
+            float px, py;
+            project(T, wx, wy, wz, s.opengl, s.H, s.W, px, py);
+            bb_min_x = std::fmin(bb_min_x, px);
+            bb_min_y = std::fmin(bb_min_y, py);
+            bb_max_x = std::fmax(bb_max_x, px);
+            bb_max_y = std::fmax(bb_max_y, py);
+            if (s.partial)
+            {
+                // VH:358-362
+                int64_t rx = f2i64(std::round(px)), ry = f2i64(std::round(py));
+                if (!in_image(ry, rx, s.H, s.W, 1))
+                    fully_inside = false;
+            }
+        }
+        // VH:236-245 (roundf rounds halves away from zero, as std::round)
+        int64_t min_rx = f2i64(std::round(bb_min_x + -0.5f));
+        int64_t min_ry = f2i64(std::round(bb_min_y + -0.5f));
+        int64_t max_rx = f2i64(std::round(bb_max_x + 0.5f));
+        int64_t max_ry = f2i64(std::round(bb_max_y + 0.5f));
+        int64_t min_bx = clamp64(min_rx, 0, s.W - 1), min_by = clamp64(min_ry, 0, s.H - 1);
+        int64_t max_bx = clamp64(max_rx, 0, s.W - 1), max_by = clamp64(max_ry, 0, s.H - 1);
+        int64_t area = (max_by - min_by + 1) * (max_bx - min_bx + 1);
+        const float* I = s.integral + b * s.H * s.W;
+        // VH:249-257
+        float i00 = sample_zero(I, s.H, s.W, min_by - 1, min_bx - 1);
+        float i10 = sample_zero(I, s.H, s.W, max_by, min_bx - 1);
+        float i01 = sample_zero(I, s.H, s.W, min_by - 1, max_bx);
+        float i11 = sample_zero(I, s.H, s.W, max_by, max_bx);
+        float sum = i11 + i00 - i10 - i01;
+        bool empty_here = false;
+        if (!s.partial)
+        {
+            // VH:269-280
+            if (sum <= 0.0f + margin)
+                empty_here = true;
+            else if (sum >= static_cast<float>(area) - margin)
+            {
+            }
+            else
+                should_refine = true;
+        }
+        else
+        {
+            // VH:376-412 (area and full area in int64 arithmetic; the unclamped box
+            // uses wrapping arithmetic for saturated corners, as on the GPU)
+            int64_t full_area = static_cast<int64_t>(
+                    (static_cast<uint64_t>(max_ry) - static_cast<uint64_t>(min_ry) + 1u) *
+                    (static_cast<uint64_t>(max_rx) - static_cast<uint64_t>(min_rx) + 1u));
+            if (sum <= 0.0f + margin && area == full_area)
+                empty_here = true;
+            else if (sum >= static_cast<float>(area) - margin && area == full_area)
+            {
+            }
+            else if (max_by - min_by > 1 && max_bx - min_bx > 1)
+                should_refine = true;
+            fully_inside_one_frame = fully_inside_one_frame || fully_inside;
+        }
+        if (empty_here)
+        {
+            is_empty = true;
+            if (!k_set)
+            {
+                k = b + 1;
+                k_set = true;
+            }
+        }
+    }
+    if (!s.partial)
+        return should_refine && !is_empty; // VH:283
+    return should_refine && !is_empty && (!last_level || fully_inside_one_frame); // VH:415
+}
+
+// Field value of one sparse grid vertex: VH:419-520 and the camera loop VH:847.
+float
+field_value(const Scene& s, int64_t gx, int64_t gy, int64_t gz, int64_t res_cells)
+{
+    float value = 1.0f; // VH:828
+    float wx = grid_to_world(gx, res_cells, s.length, s.corner[0]);
+    float wy = grid_to_world(gy, res_cells, s.length, s.corner[1]);
+    float wz = grid_to_world(gz, res_cells, s.length, s.corner[2]);
+    for (int64_t b = 0; b < s.B; ++b)
+    {
+        float px, py;
+        project(s.transforms + 16 * b, wx, wy, wz, s.opengl, s.H, s.W, px, py);
+        const float* M = s.masks + b * s.H * s.W;
+        if (s.partial)
+        {
+            // VH:513-518
+            int64_t rx = f2i64(std::round(px)), ry = f2i64(std::round(py));
+            if (!in_image(ry, rx, s.H, s.W, 1))
+                continue;
+        }
+        // IU:162-214
+        int64_t y0 = f2i64(std::round(py + -0.5f));
+        int64_t x0 = f2i64(std::round(px + -0.5f));
+        float v00, v10, v01, v11;
+        if (s.partial)
+        {
+            v00 = sample_one(M, s.H, s.W, y0, x0);
+            v10 = sample_one(M, s.H, s.W, y0 + 1, x0);
+            v01 = sample_one(M, s.H, s.W, y0, x0 + 1);
+            v11 = sample_one(M, s.H, s.W, y0 + 1, x0 + 1);
+        }
+        else
+        {
+            v00 = sample_zero(M, s.H, s.W, y0, x0);
+            v10 = sample_zero(M, s.H, s.W, y0 + 1, x0);
+            v01 = sample_zero(M, s.H, s.W, y0, x0 + 1);
+            v11 = sample_zero(M, s.H, s.W, y0 + 1, x0 + 1);
+        }
+        float ty = py - static_cast<float>(y0);
+        float tx = px - static_cast<float>(x0);
+        value *= bilerpf(v00, v10, v01, v11, ty, tx);
+    }
+    return value;
+}
+
+// ---------------------------------------------------------------------------
+// Marching cubes (sparse and dense): MC:72-112, MC:180-374, MC:535-707
+// ---------------------------------------------------------------------------
+
+// Case corner b -> (dx, dy, dz); matches index_table (marching_cubes_table.h:342)
+// composed with cube_vertex (marching_cubes_utils.h:41-45).
+constexpr int kCornerOffset[8][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { 1, 0, 1 }, { 0, 0, 1 },
+                                      { 0, 1, 0 }, { 1, 1, 0 }, { 1, 1, 1 }, { 0, 1, 1 } };
+constexpr int kEdgeCorners[12][2] = { { 0, 1 }, { 1, 2 }, { 2, 3 }, { 3, 0 }, { 4, 5 }, { 5, 6 },
+                                      { 6, 7 }, { 7, 4 }, { 0, 4 }, { 1, 5 }, { 2, 6 }, { 3, 7 } };
+
+// MC:72-97
+inline void
+interpolate_vertex(float iso, const int64_t p1[3], const int64_t p2[3], float v1, float v2, float out[3])
+{
+    const float eps = 1e-5f;
+    if (std::fabs(iso - v1) < eps)
+    {
+        for (int a = 0; a < 3; ++a)
+            out[a] = static_cast<float>(p1[a]);
+        return;
+    }
+    if (std::fabs(iso - v2) < eps)
+    {
+        for (int a = 0; a < 3; ++a)
+            out[a] = static_cast<float>(p2[a]);
+        return;
+    }
+    if (std::fabs(v1 - v2) < eps)
+    {
+        for (int a = 0; a < 3; ++a)
+            out[a] = static_cast<float>(p1[a]);
+        return;
+    }
+    float t = (iso - v1) / (v2 - v1);
+    for (int a = 0; a < 3; ++a)
+        out[a] = lerpf(static_cast<float>(p1[a]), static_cast<float>(p2[a]), t);
+}
+
+struct Mesh
+{
+    std::vector<float> verts;   // [V,3]
+    std::vector<int64_t> faces; // [F,3]
+};
+
+// Shared soup generation and unique_verts step (MC:308-374, MC:674-693).
+// `cells` lists the active cells in emission order with their 8 corner values
+// (as float, MC:343) and case index.
+struct ActiveCell
+{
+    int64_t g[3];
+    float val[8]; // by case corner
+    int ci;
+};
+
+void
+cells_to_mesh(const std::vector<ActiveCell>& cells, const int64_t full[3], float iso, bool unique, Mesh& mesh)
+{
+    int64_t nsoup = 0;
+    for (const auto& c : cells)
+        nsoup += kCases.count[c.ci];
+    std::vector<float> soup(3 * nsoup);
+    std::vector<int64_t> keys(unique ? nsoup : 0);
+    int64_t at = 0;
+    for (const auto& c : cells)
+    {
+        for (int k = 0; k < kCases.count[c.ci]; ++k, ++at)
+        {
+            int e = kCases.edges[c.ci][k];
+            int a = kEdgeCorners[e][0], b = kEdgeCorners[e][1];
+            int64_t pa[3], pb[3];
+            for (int d = 0; d < 3; ++d)
+            {
+                pa[d] = c.g[d] + kCornerOffset[a][d];
+                pb[d] = c.g[d] + kCornerOffset[b][d];
+            }
+            // The reference's edge_to_vertex_table lists the lower endpoint first
+            // (marching_cubes_table.h:323-326); reorder the case-corner pair likewise.
+            float va = c.val[a], vb = c.val[b];
+            bool swap = (pa[0] + pa[1] + pa[2]) > (pb[0] + pb[1] + pb[2]);
+            const int64_t* p1 = swap ? pb : pa;
+            const int64_t* p2 = swap ? pa : pb;
+            float v1 = swap ? vb : va, v2 = swap ? va : vb;
+            interpolate_vertex(iso, p1, p2, v1, v2, &soup[3 * at]);
+            if (unique)
+            {
+                int dir = (p2[0] != p1[0]) ? 1 : (p2[1] != p1[1]) ? 2 : 4; // edge_direction_table
+                int64_t id = p1[0] + p1[1] * full[0] + p1[2] * full[0] * full[1];
+                keys[at] = static_cast<int64_t>(static_cast<uint64_t>(id) << 3 | static_cast<uint64_t>(dir)); // MC:99-112
+            }
+        }
+    }
+    if (!unique)
+    {
+        mesh.verts = std::move(soup);
+        mesh.faces.resize(nsoup);
+        for (int64_t i = 0; i < nsoup; ++i)
+            mesh.faces[i] = i;
+        return;
+    }
+    // torch::unique_dim(sorted=true, return_inverse=true) + index_put_ (MC:677-687)
+    std::vector<int64_t> sorted_keys(keys);
+    std::sort(sorted_keys.begin(), sorted_keys.end());
+    sorted_keys.erase(std::unique(sorted_keys.begin(), sorted_keys.end()), sorted_keys.end());
+    mesh.verts.assign(3 * sorted_keys.size(), 0.0f);
+    std::vector<char> written(sorted_keys.size(), 0);
+    mesh.faces.resize(nsoup);
+    for (int64_t i = 0; i < nsoup; ++i)
+    {
+        int64_t inv = std::lower_bound(sorted_keys.begin(), sorted_keys.end(), keys[i]) - sorted_keys.begin();
+        mesh.faces[i] = inv;
+        if (written[inv])
+        {
+            // duplicates of one edge are bit-identical (same endpoints and values)
+            if (std::memcmp(&mesh.verts[3 * inv], &soup[3 * i], 3 * sizeof(float)) != 0)
+                std::abort();
+        }
+        std::memcpy(&mesh.verts[3 * inv], &soup[3 * i], 3 * sizeof(float));
+        written[inv] = 1;
+    }
+}
+
+// Sparse MC over a list of vertex ids ravelled as x + y*full[0] + z*full[0]*full[1]
+// with values; cells are visited in list order (MC:180-240, MC:308-374).
+template <typename V>
+void
+mc_sparse(const int64_t* ids, const V* values, int64_t N, const int64_t full[3], float iso, bool unique, Mesh& mesh)
+{
+    std::unordered_map<int64_t, int64_t> map;
+    map.reserve(2 * N + 1);
+    for (int64_t i = 0; i < N; ++i)
+        map.emplace(ids[i], i); // first occurrence wins, as a single-threaded emplace
+    std::vector<ActiveCell> cells;
+    for (int64_t i = 0; i < N; ++i)
+    {
+        int64_t id = ids[i];
+        int64_t g[3] = { id % full[0], id / full[0] % full[1], id / (full[0] * full[1]) };
+        if (!(g[0] < full[0] - 1 && g[1] < full[1] - 1 && g[2] < full[2] - 1))
+            continue; // MC:199, cubeindex 0
+        int ci = 0;
+        ActiveCell c{};
+        bool complete = true;
+        for (int b = 0; b < 8; ++b)
+        {
+            int64_t v = (g[0] + kCornerOffset[b][0]) + (g[1] + kCornerOffset[b][1]) * full[0] +
+                        (g[2] + kCornerOffset[b][2]) * full[0] * full[1];
+            auto it = map.find(v);
+            if (it == map.end())
+            {
+                complete = false;
+                break;
+            }
+            V val = values[it->second];
+            if (val < static_cast<V>(iso)) // native-dtype compare (MC:212)
+                ci |= 1 << b;
+            c.val[b] = static_cast<float>(val); // MC:343
+        }
+        if (!complete || kCases.count[ci] == 0)
+            continue;
+        for (int d = 0; d < 3; ++d)
+            c.g[d] = g[d];
+        c.ci = ci;
+        cells.push_back(c);
+    }
+    cells_to_mesh(cells, full, iso, unique, mesh);
+}
+
+// Dense MC on volume[D][H][W] (MC:146-178, MC:242-306).  As in the reference,
+// x pairs with size(1), y with size(2) and z with size(3) for the cell grid,
+// while the volume is read as volume[z][y][x]; only cubic volumes are
+// consistent, and the caller checks that.
+template <typename V>
+void
+mc_dense(const V* vol, int64_t S, float iso, bool unique, Mesh& mesh)
+{
+    const int64_t full[3] = { S, S, S };
+    std::vector<ActiveCell> cells;
+    for (int64_t z = 0; z + 1 < S; ++z)
+        for (int64_t y = 0; y + 1 < S; ++y)
+            for (int64_t x = 0; x + 1 < S; ++x)
+            {
+                ActiveCell c{};
+                int ci = 0;
+                for (int b = 0; b < 8; ++b)
+                {
+                    V val = vol[((z + kCornerOffset[b][2]) * S + (y + kCornerOffset[b][1])) * S +
+                                (x + kCornerOffset[b][0])];
+                    if (val < static_cast<V>(iso))
+                        ci |= 1 << b;
+                    c.val[b] = static_cast<float>(val);
+                }
+                if (kCases.count[ci] == 0)
+                    continue;
+                c.g[0] = x;
+                c.g[1] = y;
+                c.g[2] = z;
+                c.ci = ci;
+                cells.push_back(c);
+            }
+    cells_to_mesh(cells, full, iso, unique, mesh);
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian blur (GB)
+// ---------------------------------------------------------------------------
+
+// float32 <-> IEEE binary16, round to nearest even (c10::Half semantics).
+inline uint16_t
+f32_to_f16(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t mag = x & 0x7fffffffu;
+    if (mag > 0x7f800000u)
+        return static_cast<uint16_t>(sign | 0x7e00u); // NaN
+    if (mag >= 0x477ff000u)
+        return static_cast<uint16_t>(sign | 0x7c00u); // overflow to inf (>= 65520)
+    if (mag < 0x38800000u)
+    {
+        // subnormal half or zero: value = mag_float; half ulp = 2^-24
+        float a;
+        std::memcpy(&a, &mag, 4);
+        float scaled = a * 16777216.0f; // exact multiply by 2^24
+        float r = std::nearbyint(scaled); // round-half-even in default mode
+        return static_cast<uint16_t>(sign | static_cast<uint32_t>(r));
+    }
+    uint32_t e = (mag >> 23) - 112u; // rebias 127 -> 15
+    uint32_t m = mag & 0x7fffffu;
+    uint32_t h = (e << 10) | (m >> 13);
+    uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u)))
+        h += 1u;
+    return static_cast<uint16_t>(sign | h);
+}
+
+inline float
+f16_to_f32(uint16_t h)
+{
+    uint32_t sign = (h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu;
+    uint32_t m = h & 0x3ffu;
+    float out;
+    if (e == 0)
+    {
+        float v = static_cast<float>(m) * 5.9604644775390625e-08f; // m * 2^-24, exact
+        out = sign ? -v : v;
+        return out;
+    }
+    uint32_t x;
+    if (e == 31)
+        x = sign | 0x7f800000u | (m << 13);
+    else
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    std::memcpy(&out, &x, 4);
+    return out;
+}
+
+// Arithmetic in the blurred dtype T: float and double contract
+// `acc += w * x` to one fma (nvcc default, GB:225); Half rounds every
+// operation back to binary16 (c10::Half operators compute in float).
+struct HalfT
+{
+    uint16_t bits;
+};
+
+template <typename T>
+struct BlurArith;
+
+template <>
+struct BlurArith<float>
+{
+    static float from_f32(float v) { return v; }
+    static float from_f64(double v) { return static_cast<float>(v); }
+    static float mac(float acc, float w, float x) { return std::fma(w, x, acc); }
+    static float add(float a, float b) { return a + b; }
+    static float div(float a, float b) { return a / b; }
+};
+
+template <>
+struct BlurArith<double>
+{
+    static double from_f32(float v) { return static_cast<double>(v); }
+    static double from_f64(double v) { return v; }
+    static double mac(double acc, double w, double x) { return std::fma(w, x, acc); }
+    static double add(double a, double b) { return a + b; }
+    static double div(double a, double b) { return a / b; }
+};
+
+template <>
+struct BlurArith<HalfT>
+{
+    static HalfT from_f32(float v) { return HalfT{ f32_to_f16(v) }; }
+    static HalfT from_f64(double v)
+    {
+        // double -> half directly; c10::Half(double) goes through float
+        return HalfT{ f32_to_f16(static_cast<float>(v)) };
+    }
+    static float f(HalfT h) { return f16_to_f32(h.bits); }
+    static HalfT mac(HalfT acc, HalfT w, HalfT x)
+    {
+        HalfT prod = from_f32(f(w) * f(x));
+        return from_f32(f(acc) + f(prod));
+    }
+    static HalfT add(HalfT a, HalfT b) { return from_f32(f(a) + f(b)); }
+    static HalfT div(HalfT a, HalfT b) { return from_f32(f(a) / f(b)); }
+};
+
+// Input pixel values are read in double (exact for every supported input
+// dtype) and converted to the blurred dtype, like
+// static_cast<BlurredImageT>(images[...]) at GB:221.
+template <typename T>
+inline T
+to_blur(double v)
+{
+    return BlurArith<T>::from_f64(v);
+}
+
+// IU:129-160
+inline int64_t
+reflect_index(int64_t i, int64_t n)
+{
+    if (n == 1)
+        return 0;
+    int64_t p = n - 1;
+    int64_t flips = i / p + (i < 0 ? 1 : 0);
+    int64_t rem = i % p;
+    if (rem < 0)
+        rem += p;
+    return (flips % 2 == 0) ? rem : p - rem;
+}
+
+// Weight of tap (i, j) as the reference evaluates it (GB:222-223):
+// expf(-float(i*i + j*j) / (2 * sigma * sigma)) in float32; the exponential
+// is taken correctly rounded (double exp rounded to float).
+inline float
+gauss_weight(int i, int j, float sigma)
+{
+    float num = -static_cast<float>(i * i + j * j);
+    float den = 2.0f * sigma;
+    den = den * sigma;
+    float arg = num / den;
+    return static_cast<float>(std::exp(static_cast<double>(arg)));
+}
+
+template <typename T>
+void
+blur_sparse(const double* img, int64_t B, int64_t H, int64_t W, int k, float sigma, T* out)
+{
+    const int64_t n = B * H * W;
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = to_blur<T>(img[i]); // images.clone().to(dtype) (GB:444)
+    if (k == 1)
+        return;
+    const int64_t ts = k / 2;                                 // GB:384
+    const int64_t TW = W / ts + 1, TH = H / ts + 1;           // GB:155
+    std::vector<uint8_t> edge_tile(B * TH * TW, 0), band(B * TH * TW, 0);
+    // GB:116-142: edge if the right or the bottom neighbour differs
+    for (int64_t b = 0; b < B; ++b)
+        for (int64_t y = 0; y < H; ++y)
+            for (int64_t x = 0; x < W; ++x)
+            {
+                const double* I = img + b * H * W;
+                double c = I[y * W + x];
+                double r = (x + 1 < W) ? I[y * W + x + 1] : c;
+                double d = (y + 1 < H) ? I[(y + 1) * W + x] : c;
+                if (r != c || d != c)
+                    edge_tile[(b * TH + y / ts) * TW + x / ts] = 1;
+            }
+    // GB:144-178: 3x3 tile window, clamped
+    for (int64_t b = 0; b < B; ++b)
+        for (int64_t ty = 0; ty < TH; ++ty)
+            for (int64_t tx = 0; tx < TW; ++tx)
+            {
+                if (!edge_tile[(b * TH + ty) * TW + tx])
+                    continue;
+                for (int di = -1; di <= 1; ++di)
+                    for (int dj = -1; dj <= 1; ++dj)
+                    {
+                        int64_t bx = clamp64(tx + di, 0, W / ts), by = clamp64(ty + dj, 0, H / ts);
+                        band[(b * TH + by) * TW + bx] = 1;
+                    }
+            }
+    const int h = k / 2;
+    std::vector<T> w(k * k);
+    for (int i = -h; i <= h; ++i)
+        for (int j = -h; j <= h; ++j)
+            w[(i + h) * k + (j + h)] = BlurArith<T>::from_f32(gauss_weight(i, j, sigma));
+    // GB:180-232 per band pixel
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < B * TH * TW; ++t)
+    {
+        if (!band[t])
+            continue;
+        int64_t b = t / (TH * TW), ty = t / TW % TH, tx = t % TW;
+        const double* I = img + b * H * W;
+        for (int64_t ly = 0; ly < ts; ++ly)
+            for (int64_t lx = 0; lx < ts; ++lx)
+            {
+                int64_t y = ty * ts + ly, x = tx * ts + lx;
+                if (!in_image(y, x, H, W))
+                    continue;
+                T acc = BlurArith<T>::from_f32(0.0f), norm = BlurArith<T>::from_f32(0.0f);
+                for (int i = -h; i <= h; ++i)
+                    for (int j = -h; j <= h; ++j)
+                    {
+                        T v = to_blur<T>(I[reflect_index(y + j, H) * W + reflect_index(x + i, W)]);
+                        T wt = w[(i + h) * k + (j + h)];
+                        acc = BlurArith<T>::mac(acc, wt, v);
+                        norm = BlurArith<T>::add(norm, wt);
+                    }
+                out[(b * H + y) * W + x] = BlurArith<T>::div(acc, norm);
+            }
+    }
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI for the Python test harness
+// ===========================================================================
+
+extern "C" {
+
+void
+or_integral_image(const float* masks, int64_t B, int64_t H, int64_t W, float* out)
+{
+    integral_image(masks, B, H, W, out);
+}
+
+struct OrField
+{
+    int64_t nlevels;                          // number of candidate lists (level+1 unless carving stopped early)
+    std::vector<std::vector<int64_t>> levels; // candidate ravel ids per level (VH:627-751)
+    std::vector<int64_t> tests;               // camera tests per level with early exit (sum of k)
+    std::vector<int64_t> children;            // children tested per level
+    std::vector<int64_t> ids;                 // sorted unique grid-vertex ids (VH:770-825)
+    std::vector<float> values;                // field values (VH:827-888)
+};
+
+// Sparse visual-hull field: VH:582-894.
+OrField*
+or_sparse_field(const float* masks,
+                const float* integral,
+                const float* transforms,
+                int64_t B,
+                int64_t H,
+                int64_t W,
+                int level,
+                const float* corner,
+                float length,
+                int partial,
+                int opengl)
+{
+    Scene s{ masks, integral, transforms, B, H, W, opengl != 0, partial != 0, { corner[0], corner[1], corner[2] }, length };
+    auto* f = new OrField{};
+    std::vector<int64_t> cand{ 0 };
+    f->levels.push_back(cand);
+    for (int i = 0; i < level; ++i)
+    {
+        const int64_t R = int64_t{ 1 } << i, RC = int64_t{ 1 } << (i + 1);
+        const int64_t N = 8 * static_cast<int64_t>(cand.size());
+        std::vector<uint8_t> occ(N);
+        std::vector<int64_t> kk(N);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t t = 0; t < N; ++t)
+        {
+            int64_t id = cand[t / 8];
+            int c = static_cast<int>(t % 8);
+            int64_t gx = id % R, gy = id / R % R, gz = id / (R * R);
+            int64_t cx = 2 * gx + (c & 1), cy = 2 * gy + (c >> 1 & 1), cz = 2 * gz + (c >> 2 & 1);
+            occ[t] = classify_child(s, cx, cy, cz, RC, i == level - 1, kk[t]) ? 1 : 0;
+        }
+        std::vector<int64_t> next;
+        int64_t tests = 0;
+        for (int64_t t = 0; t < N; ++t)
+        {
+            tests += kk[t];
+            if (!occ[t])
+                continue;
+            int64_t id = cand[t / 8];
+            int c = static_cast<int>(t % 8);
+            int64_t gx = id % R, gy = id / R % R, gz = id / (R * R);
+            int64_t cx = 2 * gx + (c & 1), cy = 2 * gy + (c >> 1 & 1), cz = 2 * gz + (c >> 2 & 1);
+            next.push_back(cx + cy * RC + cz * RC * RC); // VH:694-711
+        }
+        f->tests.push_back(tests);
+        f->children.push_back(N);
+        cand.swap(next);
+        f->levels.push_back(cand);
+        if (cand.empty())
+            break; // VH:747-750
+    }
+    f->nlevels = static_cast<int64_t>(f->levels.size());
+    if (cand.empty())
+        return f;
+    const int64_t RC = int64_t{ 1 } << level, RG = RC + 1;
+    std::vector<int64_t> corners(8 * cand.size());
+    for (size_t t = 0; t < corners.size(); ++t)
+    {
+        int64_t id = cand[t / 8];
+        int c = static_cast<int>(t % 8);
+        int64_t gx = id % RC + (c & 1), gy = id / RC % RC + (c >> 1 & 1), gz = id / (RC * RC) + (c >> 2 & 1);
+        corners[t] = gx + gy * RG + gz * RG * RG; // VH:776-789
+    }
+    std::sort(corners.begin(), corners.end());
+    corners.erase(std::unique(corners.begin(), corners.end()), corners.end());
+    f->ids = corners;
+    f->values.resize(corners.size());
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < static_cast<int64_t>(corners.size()); ++i)
+    {
+        int64_t id = corners[i];
+        f->values[i] = field_value(s, id % RG, id / RG % RG, id / (RG * RG), RC);
+    }
+    return f;
+}
+
+int64_t
+or_field_nlevels(const OrField* f)
+{
+    return f->nlevels;
+}
+
+int64_t
+or_field_level_size(const OrField* f, int64_t i)
+{
+    return static_cast<int64_t>(f->levels[i].size());
+}
+
+void
+or_field_level_copy(const OrField* f, int64_t i, int64_t* out)
+{
+    std::memcpy(out, f->levels[i].data(), f->levels[i].size() * sizeof(int64_t));
+}
+
+int64_t
+or_field_level_tests(const OrField* f, int64_t i)
+{
+    return f->tests[i];
+}
+
+int64_t
+or_field_nverts(const OrField* f)
+{
+    return static_cast<int64_t>(f->ids.size());
+}
+
+void
+or_field_copy(const OrField* f, int64_t* ids, float* values)
+{
+    std::memcpy(ids, f->ids.data(), f->ids.size() * sizeof(int64_t));
+    std::memcpy(values, f->values.data(), f->values.size() * sizeof(float));
+}
+
+void
+or_field_free(OrField* f)
+{
+    delete f;
+}
+
+struct OrMesh
+{
+    Mesh m;
+};
+
+int64_t
+or_mesh_nverts(const OrMesh* m)
+{
+    return static_cast<int64_t>(m->m.verts.size() / 3);
+}
+
+int64_t
+or_mesh_nfaces(const OrMesh* m)
+{
+    return static_cast<int64_t>(m->m.faces.size() / 3);
+}
+
+void
+or_mesh_copy(const OrMesh* m, float* verts, int64_t* faces)
+{
+    std::memcpy(verts, m->m.verts.data(), m->m.verts.size() * sizeof(float));
+    std::memcpy(faces, m->m.faces.data(), m->m.faces.size() * sizeof(int64_t));
+}
+
+void
+or_mesh_free(OrMesh* m)
+{
+    delete m;
+}
+
+// Sparse MC over ravelled ids (x + y*sx + z*sx*sy), values float32 (is_f64=0)
+// or float64 (is_f64=1), local grid coordinates (return_local_coords=false).
+OrMesh*
+or_marching_cubes_sparse(const int64_t* ids,
+                         const void* values,
+                         int is_f64,
+                         int64_t N,
+                         int64_t sx,
+                         int64_t sy,
+                         int64_t sz,
+                         float iso,
+                         int unique)
+{
+    auto* m = new OrMesh{};
+    const int64_t full[3] = { sx, sy, sz };
+    if (is_f64)
+        mc_sparse(ids, static_cast<const double*>(values), N, full, iso, unique != 0, m->m);
+    else
+        mc_sparse(ids, static_cast<const float*>(values), N, full, iso, unique != 0, m->m);
+    return m;
+}
+
+OrMesh*
+or_marching_cubes_dense(const void* volume, int is_f64, int64_t S, float iso, int unique)
+{
+    auto* m = new OrMesh{};
+    if (is_f64)
+        mc_dense(static_cast<const double*>(volume), S, iso, unique != 0, m->m);
+    else
+        mc_dense(static_cast<const float*>(volume), S, iso, unique != 0, m->m);
+    return m;
+}
+
+// Visual hull: field -> sparse MC at isolevel 0.5 in local grid units ->
+// global coordinates and face flip (visual_hull.cpp:34-58, VH:916-1020).
+OrMesh*
+or_visual_hull(const OrField* f, int level, const float* corner, float length, int unique)
+{
+    auto* m = new OrMesh{};
+    const int64_t RG = (int64_t{ 1 } << level) + 1;
+    const int64_t full[3] = { RG, RG, RG };
+    mc_sparse(f->ids.data(), f->values.data(), static_cast<int64_t>(f->ids.size()), full, 0.5f, unique != 0, m->m);
+    const float scale = length / static_cast<float>(int64_t{ 1 } << level);
+    for (size_t i = 0; i < m->m.verts.size(); ++i)
+        m->m.verts[i] = std::fma(scale, m->m.verts[i], corner[i % 3]); // VH:929-937
+    for (size_t i = 0; i < m->m.faces.size(); i += 3)
+        std::swap(m->m.faces[i + 1], m->m.faces[i + 2]); // VH:951-954
+    return m;
+}
+
+// Sparse Gaussian blur.  images: float64 [B,H,W] holding the input values
+// (exact for every input dtype the API accepts); out_dtype 0=f16 (out is
+// uint16 bits), 1=f32, 2=f64.
+void
+or_gaussian_blur_sparse(const double* images, int64_t B, int64_t H, int64_t W, int k, float sigma, int out_dtype, void* out)
+{
+    if (out_dtype == 0)
+        blur_sparse(images, B, H, W, k, sigma, static_cast<HalfT*>(out));
+    else if (out_dtype == 1)
+        blur_sparse(images, B, H, W, k, sigma, static_cast<float*>(out));
+    else
+        blur_sparse(images, B, H, W, k, sigma, static_cast<double*>(out));
+}
+
+// Weight table of the blur as evaluated above, for tests: k*k floats, index (i+h)*k+(j+h).
+void
+or_gaussian_weights(int k, float sigma, float* out)
+{
+    const int h = k / 2;
+    for (int i = -h; i <= h; ++i)
+        for (int j = -h; j <= h; ++j)
+            out[(i + h) * k + (j + h)] = gauss_weight(i, j, sigma);
+}
+
+void
+or_f32_to_f16(const float* in, int64_t n, uint16_t* out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = f32_to_f16(in[i]);
+}
+
+} // extern "C"

@@ -1,0 +1,206 @@
+"""ctypes front-end of the CPU parity oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by bench.py's CPU-baseline leg, never by the product package.  The oracle
+restates the reference algorithm on the CPU with the reference's floating-point
+operation order; see oracle.cpp for the per-function citations.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "liboracle.so"
+_lib = None
+
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return a.ctypes.data_as(c_void_p)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIB_PATH))
+        L.or_sparse_field.restype = c_void_p
+        L.or_sparse_field.argtypes = [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_i64, c_int, c_void_p, c_float,
+                                      c_int, c_int]
+        for name in ("or_field_nlevels", "or_field_nverts", "or_mesh_nverts", "or_mesh_nfaces"):
+            getattr(L, name).restype = c_i64
+            getattr(L, name).argtypes = [c_void_p]
+        L.or_field_level_size.restype = c_i64
+        L.or_field_level_size.argtypes = [c_void_p, c_i64]
+        L.or_field_level_tests.restype = c_i64
+        L.or_field_level_tests.argtypes = [c_void_p, c_i64]
+        L.or_field_level_copy.argtypes = [c_void_p, c_i64, c_void_p]
+        L.or_field_copy.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.or_field_free.argtypes = [c_void_p]
+        L.or_mesh_copy.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.or_mesh_free.argtypes = [c_void_p]
+        L.or_integral_image.argtypes = [c_void_p, c_i64, c_i64, c_i64, c_void_p]
+        L.or_marching_cubes_sparse.restype = c_void_p
+        L.or_marching_cubes_sparse.argtypes = [c_void_p, c_void_p, c_int, c_i64, c_i64, c_i64, c_i64, c_float, c_int]
+        L.or_marching_cubes_dense.restype = c_void_p
+        L.or_marching_cubes_dense.argtypes = [c_void_p, c_int, c_i64, c_float, c_int]
+        L.or_visual_hull.restype = c_void_p
+        L.or_visual_hull.argtypes = [c_void_p, c_int, c_void_p, c_float, c_int]
+        L.or_gaussian_blur_sparse.argtypes = [c_void_p, c_i64, c_i64, c_i64, c_int, c_float, c_int, c_void_p]
+        L.or_gaussian_weights.argtypes = [c_int, c_float, c_void_p]
+        L.or_f32_to_f16.argtypes = [c_void_p, c_i64, c_void_p]
+        _lib = L
+    return _lib
+
+
+def _masks_f32(masks: np.ndarray) -> np.ndarray:
+    """[B,H,W] or [B,H,W,1] of any dtype -> contiguous float32 [B,H,W] (IU:178 casts to float)."""
+    m = np.asarray(masks)
+    if m.ndim == 4:
+        assert m.shape[3] == 1
+        m = m[..., 0]
+    return np.ascontiguousarray(m.astype(np.float32))
+
+
+def integral_image(masks: np.ndarray) -> np.ndarray:
+    """Integral image as torch::cumsum(.., 1, f32) then cumsum(.., 2, f32) on CUDA (VH:76-90)."""
+    m = _masks_f32(masks)
+    out = np.empty_like(m)
+    lib().or_integral_image(_ptr(m), m.shape[0], m.shape[1], m.shape[2], _ptr(out))
+    return out
+
+
+def _mesh_from_handle(h: int) -> tuple[np.ndarray, np.ndarray]:
+    L = lib()
+    nv = L.or_mesh_nverts(h)
+    nf = L.or_mesh_nfaces(h)
+    verts = np.empty((nv, 3), np.float32)
+    faces = np.empty((nf, 3), np.int64)
+    L.or_mesh_copy(h, _ptr(verts), _ptr(faces))
+    L.or_mesh_free(h)
+    return verts, faces
+
+
+class Field:
+    """Result of the sparse visual-hull field (VH:582-894)."""
+
+    def __init__(self, handle: int) -> None:
+        self.handle = handle
+        L = lib()
+        n = L.or_field_nlevels(handle)
+        self.candidates = []
+        self.tests = []
+        for i in range(n):
+            arr = np.empty(L.or_field_level_size(handle, i), np.int64)
+            L.or_field_level_copy(handle, i, _ptr(arr))
+            self.candidates.append(arr)
+            if i + 1 < n:
+                self.tests.append(L.or_field_level_tests(handle, i))
+        nv = L.or_field_nverts(handle)
+        self.ids = np.empty(nv, np.int64)
+        self.values = np.empty(nv, np.float32)
+        L.or_field_copy(handle, _ptr(self.ids), _ptr(self.values))
+
+    def __del__(self) -> None:
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.or_field_free(self.handle)
+                self.handle = None
+        except Exception:  # interpreter shutdown
+            pass
+
+
+def sparse_field(masks: np.ndarray, transforms: np.ndarray, level: int, cube_corner_bfl, cube_length: float,
+                 masks_partial: bool, transforms_convention: str) -> Field:
+    m = _masks_f32(masks)
+    T = np.ascontiguousarray(np.asarray(transforms).astype(np.float32).reshape(m.shape[0], 16))
+    integral = integral_image(m)
+    corner = np.ascontiguousarray(np.asarray(cube_corner_bfl, np.float32).reshape(3))
+    if transforms_convention not in ("opengl", "opencv"):
+        raise RuntimeError(f'Unsupported transforms_convention "{transforms_convention}".')
+    h = lib().or_sparse_field(_ptr(m), _ptr(integral), _ptr(T), m.shape[0], m.shape[1], m.shape[2], int(level),
+                              _ptr(corner), float(cube_length), int(bool(masks_partial)),
+                              int(transforms_convention == "opengl"))
+    return Field(h)
+
+
+def visual_hull(masks, transforms, level, cube_corner_bfl, cube_length, masks_partial, transforms_convention,
+                unique_verts=True, field: Field | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """Visual hull mesh (visual_hull.cpp:34-58): verts float32 [V,3], faces int64 [F,3]."""
+    if field is None:
+        field = sparse_field(masks, transforms, level, cube_corner_bfl, cube_length, masks_partial,
+                             transforms_convention)
+    corner = np.ascontiguousarray(np.asarray(cube_corner_bfl, np.float32).reshape(3))
+    h = lib().or_visual_hull(field.handle, int(level), _ptr(corner), float(cube_length), int(bool(unique_verts)))
+    return _mesh_from_handle(h)
+
+
+def marching_cubes_sparse(ids: np.ndarray, values: np.ndarray, sizes, isolevel: float,
+                          unique_verts: bool = True) -> tuple[np.ndarray, np.ndarray]:
+    """Sparse MC over ravelled ids x + y*sx + z*sx*sy (MC:535-707); local grid coordinates."""
+    ids = np.ascontiguousarray(ids, np.int64)
+    is_f64 = values.dtype == np.float64
+    vals = np.ascontiguousarray(values, np.float64 if is_f64 else np.float32)
+    sx, sy, sz = (int(s) for s in sizes)
+    h = lib().or_marching_cubes_sparse(_ptr(ids), _ptr(vals), int(is_f64), ids.size, sx, sy, sz, float(isolevel),
+                                       int(bool(unique_verts)))
+    return _mesh_from_handle(h)
+
+
+def marching_cubes_dense(volume: np.ndarray, isolevel: float, unique_verts: bool = True) -> tuple[np.ndarray, np.ndarray]:
+    """Dense MC of a cubic volume [S,S,S] indexed [z][y][x] (MC:376-533); local grid coordinates."""
+    v = np.asarray(volume)
+    if v.ndim == 4:
+        v = v[0]
+    is_f64 = v.dtype == np.float64
+    v = np.ascontiguousarray(v, np.float64 if is_f64 else np.float32)
+    assert v.shape[0] == v.shape[1] == v.shape[2]
+    h = lib().or_marching_cubes_dense(_ptr(v), int(is_f64), v.shape[0], float(isolevel), int(bool(unique_verts)))
+    return _mesh_from_handle(h)
+
+
+_OUT_CODES = {np.float16: 0, np.float32: 1, np.float64: 2}
+
+
+def gaussian_blur_sparse(images: np.ndarray, kernel_size: int, sigma: float, out_dtype=np.float32) -> np.ndarray:
+    """Sparse band blur (GB:288-507). images [B,H,W] or [B,H,W,1]; returns same shape in out_dtype."""
+    img = np.asarray(images)
+    shape = img.shape
+    if img.ndim == 4:
+        img = img[..., 0]
+    src = np.ascontiguousarray(img.astype(np.float64))
+    code = _OUT_CODES[np.dtype(out_dtype).type]
+    out = np.empty(src.shape, np.uint16 if code == 0 else out_dtype)
+    lib().or_gaussian_blur_sparse(_ptr(src), src.shape[0], src.shape[1], src.shape[2], int(kernel_size),
+                                  float(sigma), code, _ptr(out))
+    if code == 0:
+        out = out.view(np.float16)
+    return out.reshape(shape)
+
+
+def gaussian_weights(kernel_size: int, sigma: float) -> np.ndarray:
+    w = np.empty((kernel_size, kernel_size), np.float32)
+    lib().or_gaussian_weights(int(kernel_size), float(sigma), _ptr(w))
+    return w
+
+
+def f32_to_f16(x: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty(a.shape, np.uint16)
+    lib().or_f32_to_f16(_ptr(a), a.size, _ptr(out))
+    return out.view(np.float16)

@@ -1,0 +1,281 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Bar: integer/index results bit-exact (integral image of binary masks,
+candidate lists, sparse field indices, faces); float results bit-exact where
+the oracle follows the same operation order (integral image, field values,
+blur), vertices within 1e-6 * cube_length.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle
+from tests.gpu_util import DEV, assert_mesh_equal, is_closed_manifold
+from torchhull_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _th():
+    import torchhull_amd as th
+
+    return th
+
+
+def _lib():
+    from torchhull_amd import _lib
+
+    return _lib
+
+
+# ---------------------------------------------------------------------------
+# integral image
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("kind", ["binary", "fractional"])
+@pytest.mark.parametrize("shape", [(3, 17, 33), (2, 256, 300), (1, 1024, 64)])
+def test_integral_image_matches_oracle_and_torch(kind, shape):
+    rng = np.random.default_rng(1)
+    if kind == "binary":
+        m = (rng.uniform(size=shape) < 0.4).astype(np.float32)
+    else:
+        m = rng.uniform(size=shape).astype(np.float32)
+    lib = _lib()
+    dm = torch.from_numpy(m).to(DEV)
+    out = torch.empty_like(dm)
+    ctx = lib.context(0)
+    lib.check(lib.lib().thull_integral_image(ctx, dm.data_ptr(), lib.F32, *shape, out.data_ptr(),
+                                             torch.cuda.current_stream().cuda_stream))
+    got = out.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), oracle.integral_image(m).view(np.uint32))
+    # ATen's cumsum over outer dimensions on this GPU (tensor_kernel_scan_outer_dim) gives the same bits
+    ref = torch.cumsum(torch.cumsum(dm[..., None], 1, dtype=torch.float32), 2, dtype=torch.float32)[..., 0]
+    assert torch.equal(out.view(torch.int32), ref.contiguous().view(torch.int32))
+
+
+# ---------------------------------------------------------------------------
+# visual hull
+# ---------------------------------------------------------------------------
+
+SCENES = {
+    "c0": (0, {"level": 7}),
+    "c0_l5": (0, {"level": 5}),
+    "c1_small": (1, {"n_cameras": 12, "resolution": 256, "level": 7}),
+    "c2_small": (2, {"n_cameras": 12, "resolution": 384, "level": 7}),
+    "c2_gl_full": (2, {"n_cameras": 10, "resolution": 256, "level": 6, "convention": "opengl",
+                       "masks_partial": False}),
+    "c0_partial": (0, {"level": 6, "masks_partial": True}),
+}
+
+
+def _scene(name):
+    cfg, over = SCENES[name]
+    return synthetic.config_scene(cfg, 0, device="cpu", spec_override=over)
+
+
+def _blurred(sc):
+    """Masks after the scene's blur (GPU), as float32 numpy for the oracle and the device tensor."""
+    th = _th()
+    m = sc.masks.to(DEV)
+    if sc.blur is not None:
+        k, s = sc.blur
+        m = th.gaussian_blur(m, k, s, sparse=True, dtype=torch.float32)
+        ob = oracle.gaussian_blur_sparse(sc.masks.numpy(), k, s, np.float32)
+        assert np.array_equal(m.cpu().numpy().view(np.uint32), ob.view(np.uint32)), "blurred masks differ"
+    return m
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("unique_verts", [True, False])
+def test_visual_hull_matches_oracle(name, unique_verts):
+    th = _th()
+    sc = _scene(name)
+    m = _blurred(sc)
+    T = sc.transforms.to(DEV)
+    print(f"case {name} unique={unique_verts} B={sc.num_cameras} H={m.shape[1]} L={sc.level}", flush=True)
+    v, f, cand = th.visual_hull_with_candidate_voxels(m, T, sc.level, sc.cube_corner_bfl, sc.cube_length,
+                                                      sc.masks_partial, sc.transforms_convention, unique_verts)
+    field = oracle.sparse_field(m.cpu().numpy(), sc.transforms.numpy(), sc.level, sc.cube_corner_bfl,
+                                sc.cube_length, sc.masks_partial, sc.transforms_convention)
+    # candidate lists of every level, in carving order
+    assert len(cand) == len(field.candidates)
+    for a, b in zip(cand, field.candidates):
+        assert np.array_equal(a.cpu().numpy(), b)
+    ov, of = oracle.visual_hull(None, None, sc.level, sc.cube_corner_bfl, sc.cube_length, sc.masks_partial,
+                                sc.transforms_convention, unique_verts, field=field)
+    assert_mesh_equal(v, f, ov, of, atol=1e-6 * sc.cube_length)
+    if unique_verts:
+        # with masks_partial, regions few cameras see stay "unknown" and the hull may reach the
+        # cube faces, where no complete cell closes it; full-mask hulls are closed
+        if not sc.masks_partial:
+            assert is_closed_manifold(f.cpu().numpy())
+    else:
+        assert v.shape[0] == 3 * f.shape[0]
+
+
+@pytest.mark.parametrize("name", ["c0", "c2_small"])
+def test_sparse_field_matches_oracle(name):
+    th = _th()
+    sc = _scene(name)
+    m = _blurred(sc)
+    T = sc.transforms.to(DEV)
+    fld = th.sparse_visual_hull_field(m, T, sc.level, sc.cube_corner_bfl, sc.cube_length, sc.masks_partial,
+                                      sc.transforms_convention)
+    ref = oracle.sparse_field(m.cpu().numpy(), sc.transforms.numpy(), sc.level, sc.cube_corner_bfl,
+                              sc.cube_length, sc.masks_partial, sc.transforms_convention)
+    R1 = (1 << sc.level) + 1
+    idx = fld.indices().cpu().numpy()
+    ids = idx[3] + idx[2] * R1 + idx[1] * R1 * R1
+    assert fld.shape == (1, R1, R1, R1)
+    assert np.array_equal(ids, ref.ids)
+    got = fld.values().cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(ref.values))
+    assert np.array_equal(got[~np.isnan(got)].view(np.uint32), ref.values[~np.isnan(ref.values)].view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [torch.uint8, torch.bool, torch.int64, torch.float16, torch.float64])
+def test_visual_hull_mask_dtypes(dtype):
+    th = _th()
+    sc = _scene("c0_l5")
+    T = sc.transforms.to(DEV).to(torch.float64)
+    m = sc.masks.to(DEV).to(dtype)
+    v, f = th.visual_hull(m, T, sc.level, sc.cube_corner_bfl, sc.cube_length, False, "opengl", True)
+    ov, of = oracle.visual_hull(sc.masks.numpy(), sc.transforms.numpy().astype(np.float64).astype(np.float32),
+                                sc.level, sc.cube_corner_bfl, sc.cube_length, False, "opengl", True)
+    assert_mesh_equal(v, f, ov, of, atol=1e-6 * sc.cube_length)
+
+
+@pytest.mark.parametrize("fill", [0.0, 1.0])
+@pytest.mark.parametrize("partial", [True, False])
+def test_visual_hull_empty_and_full_masks(fill, partial):
+    th = _th()
+    sc = _scene("c0_l5")
+    m = torch.full_like(sc.masks, fill).to(DEV)
+    for level in range(0, 7):
+        v, f = th.visual_hull(m, sc.transforms.to(DEV), level, (-1.1, -1.1, -1.1), 2.2, partial, "opengl", True)
+        assert v.shape[0] == 0 and f.shape[0] == 0
+
+
+def test_visual_hull_batch_equals_single():
+    th = _th()
+    scenes = []
+    for name in ["c0", "c0_partial", "c2_small", "c1_small"]:
+        sc = _scene(name)
+        m = _blurred(sc)
+        scenes.append(dict(masks=m, transforms=sc.transforms.to(DEV), level=sc.level,
+                           cube_corner_bfl=sc.cube_corner_bfl, cube_length=sc.cube_length,
+                           masks_partial=sc.masks_partial, transforms_convention=sc.transforms_convention))
+    batch = th.visual_hull_batch(scenes, unique_verts=True)
+    for s, (vb, fb) in zip(scenes, batch):
+        v, f = th.visual_hull(**s, unique_verts=True)
+        assert torch.equal(f, fb)
+        assert torch.equal(v.view(torch.int32), vb.view(torch.int32))
+
+
+# ---------------------------------------------------------------------------
+# marching cubes
+# ---------------------------------------------------------------------------
+
+
+def _sphere(size, dtype):
+    g = torch.arange(size, dtype=dtype)
+    z, y, x = torch.meshgrid(g, g, g, indexing="ij")
+    c = size / 2
+    r = 0.8731945 * (size / 2)
+    return (torch.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) - r).reshape(1, size, size, size)
+
+
+@pytest.mark.parametrize("size", [20, 65])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("unique_verts", [True, False])
+@pytest.mark.parametrize("local", [True, False])
+def test_marching_cubes_dense_matches_oracle(size, dtype, unique_verts, local):
+    th = _th()
+    sdf = _sphere(size, dtype)
+    v, f = th.marching_cubes(sdf.to(DEV), 0.0, return_local_coords=local, unique_verts=unique_verts)
+    ov, of = oracle.marching_cubes_dense(sdf.numpy(), 0.0, unique_verts)
+    if local:
+        s = np.float32(2.0) / np.float32(size - 1)
+        ov = (ov * s).astype(np.float32) - np.float32(1.0)
+    assert_mesh_equal(v, f, ov, of, atol=0.0)
+    if unique_verts:
+        assert is_closed_manifold(f.cpu().numpy())
+
+
+@pytest.mark.parametrize("size", [24, 80])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("unique_verts", [True, False])
+def test_marching_cubes_sparse_matches_oracle(size, dtype, unique_verts):
+    th = _th()
+    sdf = _sphere(size, dtype)
+    sdf[sdf.abs() > 2] = 0
+    sp = sdf.to_sparse()
+    v, f = th.marching_cubes(sp.to(DEV), 0.0, return_local_coords=False, unique_verts=unique_verts)
+    idx = sp.indices().numpy()
+    ids = idx[3] + idx[2] * size + idx[1] * size * size
+    ov, of = oracle.marching_cubes_sparse(ids, sp.values().numpy(), (size, size, size), 0.0, unique_verts)
+    assert_mesh_equal(v, f, ov, of, atol=0.0)
+    if unique_verts:
+        assert is_closed_manifold(f.cpu().numpy())
+    if size < 64:
+        return
+    # reference property (tests/test_marching_cubes.py:93, sizes 100-400): |sdf(v)| <= 1e-2
+    c = size / 2
+    r = 0.8731945 * (size / 2)
+    vv = v.cpu().double()
+    d = torch.linalg.norm(vv - c, dim=1) - r
+    assert d.abs().max() <= 1e-2
+
+
+def test_marching_cubes_empty():
+    th = _th()
+    for dtype in (torch.float32, torch.float64):
+        v, f = th.marching_cubes(torch.full((1, 30, 30, 30), -1.0, dtype=dtype, device=DEV), 0.0, False)
+        assert v.shape == (0, 3) and f.shape == (0, 3)
+        sp = torch.sparse_coo_tensor(torch.empty((4, 0), dtype=torch.int64), torch.empty((0,), dtype=dtype),
+                                     (1, 30, 30, 30)).to(DEV)
+        v, f = th.marching_cubes(sp, 0.0, False)
+        assert v.shape == (0, 3) and f.shape == (0, 3)
+
+
+# ---------------------------------------------------------------------------
+# gaussian blur
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("k", [1, 3, 7, 9, 15])
+@pytest.mark.parametrize("out_dtype", [torch.float16, torch.float32, torch.float64])
+@pytest.mark.parametrize("in_dtype", [torch.uint8, torch.float32, torch.float64])
+def test_sparse_blur_matches_oracle(k, out_dtype, in_dtype):
+    th = _th()
+    sc = _scene("c0_l5")
+    m = sc.masks[:3, 40:200, 30:230].clone()
+    mi = m.to(in_dtype)
+    got = th.gaussian_blur(mi.to(DEV), k, 1.7, sparse=True, dtype=out_dtype).cpu().numpy()
+    npd = {torch.float16: np.float16, torch.float32: np.float32, torch.float64: np.float64}[out_dtype]
+    ref = oracle.gaussian_blur_sparse(mi.to(torch.float64).numpy(), k, 1.7, npd)
+    assert got.dtype == ref.dtype
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("k", [3, 9, 15])
+@pytest.mark.parametrize("sigma", [1.0, 100.0])
+def test_blur_dense_and_sparse_consistent(k, sigma):
+    th = _th()
+    sc = _scene("c0_l5")
+    m = sc.masks.to(DEV)
+    dense = th.gaussian_blur(m, k, sigma, sparse=False, dtype=torch.float32)
+    sparse = th.gaussian_blur(m, k, sigma, sparse=True, dtype=torch.float32)
+    # tests/test_gaussian_blur.py:236-243 of the reference
+    assert torch.allclose(dense, sparse, atol=1e-7)
+    # dense blur against torch's conv2d with reflect padding (float64 on the CPU)
+    w = torch.from_numpy(oracle.gaussian_weights(k, sigma)).double()
+    w = w / w.sum()
+    x = torch.nn.functional.pad(sc.masks.permute(0, 3, 1, 2).double(), (k // 2,) * 4, mode="reflect")
+    ref = torch.nn.functional.conv2d(x, w.T[None, None]).permute(0, 2, 3, 1)
+    assert torch.allclose(dense.cpu().double(), ref, atol=1e-6)
+    assert torch.all(dense >= -1e-5) and torch.all(dense <= 1 + 1e-5)

@@ -1,0 +1,47 @@
+"""MI355X-native visual hulls: a drop-in for the Python API of vc-bonn/torchhull.
+
+Same function names, argument names, defaults, tensor layouts and result
+dtypes as the reference's bindings (src/torchhull/_C/python/bindings.cpp);
+the work runs in hand-written HIP kernels for gfx950 behind the C ABI of
+include/torchhull_amd.h.  Like the reference, every entry point accepts
+tensors on a GPU only and raises RuntimeError otherwise.
+
+.. code-block:: python
+
+    import torchhull_amd as torchhull
+
+    verts, faces = torchhull.visual_hull(masks,       # [B, H, W, 1]
+                                         transforms,  # [B, 4, 4]
+                                         level,
+                                         cube_corner_bfl,
+                                         cube_length,
+                                         masks_partial=False,
+                                         transforms_convention="opengl",
+                                         unique_verts=True)
+"""
+
+from __future__ import annotations
+
+from .api import (
+    candidate_voxels_to_wireframes,
+    gaussian_blur,
+    marching_cubes,
+    sparse_visual_hull_field,
+    store_curve_network,
+    visual_hull,
+    visual_hull_batch,
+    visual_hull_with_candidate_voxels,
+)
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "candidate_voxels_to_wireframes",
+    "gaussian_blur",
+    "marching_cubes",
+    "sparse_visual_hull_field",
+    "store_curve_network",
+    "visual_hull",
+    "visual_hull_batch",
+    "visual_hull_with_candidate_voxels",
+]

@@ -1,0 +1,120 @@
+"""ctypes binding of libtorchhull_hip.so (the C ABI declared in include/torchhull_amd.h).
+
+The library is built in-tree by ``torchhull_amd.build.build()``; there is no
+fallback implementation: every entry point raises if the library is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "_build" / "libtorchhull_hip.so"
+HEADER_PATH = _HERE.parent / "include" / "torchhull_amd.h"
+
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+P_i64 = ctypes.POINTER(ctypes.c_int64)
+
+# thull_dtype
+BOOL, U8, I8, I16, I32, I64, F16, F32, F64 = range(9)
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("masks", c_void_p),
+        ("mask_dtype", c_i32),
+        ("transforms", c_void_p),
+        ("B", c_i64),
+        ("H", c_i64),
+        ("W", c_i64),
+        ("level", c_i32),
+        ("masks_partial", c_i32),
+        ("opengl", c_i32),
+        ("cube_corner_bfl", c_float * 3),
+        ("cube_length", c_float),
+    ]
+
+
+class VhCounts(ctypes.Structure):
+    _fields_ = [
+        ("num_verts", c_i64),
+        ("num_faces", c_i64),
+        ("num_field", c_i64),
+        ("num_levels", c_i32),
+        ("pad_", c_i32),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.thull_abi_version.restype = ctypes.c_int
+    L.thull_abi_version.argtypes = []
+    L.thull_last_error.restype = ctypes.c_char_p
+    L.thull_last_error.argtypes = []
+    L.thull_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(c_void_p)]
+    L.thull_ctx_destroy.argtypes = [c_void_p]
+    L.thull_ctx_memory.argtypes = [c_void_p, P_i64, P_i64]
+    L.thull_ctx_trim.argtypes = [c_void_p]
+    L.thull_visual_hull_run.argtypes = [c_void_p, ctypes.POINTER(Scene), c_i32, c_i32, c_void_p,
+                                        ctypes.POINTER(VhCounts)]
+    L.thull_visual_hull_fetch_mesh.argtypes = [c_void_p, c_i32, c_void_p, c_void_p, c_void_p]
+    L.thull_visual_hull_fetch_field.argtypes = [c_void_p, c_i32, c_void_p, c_void_p, c_void_p]
+    L.thull_visual_hull_candidates_count.argtypes = [c_void_p, c_i32, P_i64]
+    L.thull_visual_hull_fetch_candidates.argtypes = [c_void_p, c_i32, c_void_p, c_void_p]
+    L.thull_visual_hull_stats.argtypes = [c_void_p, c_i32, P_i64, P_i64]
+    L.thull_marching_cubes_dense_run.argtypes = [c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_float, c_i32,
+                                                 c_i32, c_void_p, P_i64, P_i64]
+    L.thull_marching_cubes_sparse_run.argtypes = [c_void_p, c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_i64,
+                                                  c_float, c_i32, c_i32, c_void_p, P_i64, P_i64]
+    L.thull_marching_cubes_fetch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+    L.thull_gaussian_blur.argtypes = [c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_i32, c_float, c_i32,
+                                      c_void_p, c_i32, c_void_p]
+    L.thull_integral_image.argtypes = [c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_void_p, c_void_p]
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not LIB_PATH.exists():
+                    raise RuntimeError(
+                        f"torchhull_amd: HIP library {LIB_PATH} is not built; run `python -c \"import "
+                        f"__graft_entry__; __graft_entry__.build()\"` (or `make -C torchhull_amd/csrc`)")
+                L = ctypes.CDLL(str(LIB_PATH))
+                _declare(L)
+                if L.thull_abi_version() != 1:
+                    raise RuntimeError("torchhull_amd: ABI version mismatch")
+                _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = lib().thull_last_error().decode("utf-8", "replace")
+        raise RuntimeError(msg)
+
+
+_tls = threading.local()
+
+
+def context(device_index: int) -> c_void_p:
+    """Per-thread, per-device library context (pooled device memory)."""
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    ctx = ctxs.get(device_index)
+    if ctx is None:
+        ctx = c_void_p()
+        check(lib().thull_ctx_create(int(device_index), ctypes.byref(ctx)))
+        ctxs[device_index] = ctx
+    return ctx

@@ -1,0 +1,359 @@
+"""Python entry points, mirroring the reference's bindings (src/torchhull/_C/python/bindings.cpp).
+
+Argument checks raise RuntimeError with the reference's conditions
+(visual_hull_cuda.cu:591-613, marching_cubes_cuda.cu:382-385,715-719,762-768,
+gaussian_blur_cuda.cu:79-81,294-305).  Tensor plumbing (dtype conversion,
+contiguity, output allocation) uses torch; all computation runs in the HIP
+library.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import time
+from collections.abc import Sequence
+
+import torch
+
+from . import _lib
+
+_DTYPE_CODES = {
+    torch.bool: _lib.BOOL,
+    torch.uint8: _lib.U8,
+    torch.int8: _lib.I8,
+    torch.int16: _lib.I16,
+    torch.int32: _lib.I32,
+    torch.int64: _lib.I64,
+    torch.float16: _lib.F16,
+    torch.float32: _lib.F32,
+    torch.float64: _lib.F64,
+}
+
+
+def _check(cond: bool, text: str) -> None:
+    if not cond:
+        raise RuntimeError(f"Check failed: {text}")
+
+
+def _require_gpu(t: torch.Tensor) -> None:
+    # visual_hull.cpp:70-82, marching_cubes.cpp:20-25, gaussian_blur.cpp:22-27
+    if not t.is_cuda:
+        raise RuntimeError(f'No backend implementation available for device "{t.device}".')
+
+
+def _stream(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _convention(transforms_convention: str) -> int:
+    if transforms_convention == "opengl":
+        return 1
+    if transforms_convention == "opencv":
+        return 0
+    raise RuntimeError(f'Unsupported transforms_convention "{transforms_convention}".')
+
+
+def _masks_for_abi(masks: torch.Tensor) -> tuple[torch.Tensor, int]:
+    """[B,H,W,1] masks -> contiguous [B,H,W] in a dtype the kernels read natively.
+
+    float32, uint8 and bool are read as they are; every other dtype is converted
+    to float32 first, which is what the reference does per sample
+    (image_utils.h:178 casts to float) and for the integral (cumsum in float32)."""
+    m = masks[..., 0]
+    if m.dtype in (torch.float32, torch.uint8, torch.bool):
+        m = m.contiguous()
+    else:
+        m = m.to(torch.float32).contiguous()
+    code = _DTYPE_CODES[m.dtype]
+    return m, code
+
+
+def _check_vh_args(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_length: float) -> None:
+    _check(masks.device == transforms.device, "masks.device() == transforms.device()")
+    _check(masks.dim() == 4, "masks.dim() == 4")
+    _check(transforms.dim() == 3, "transforms.dim() == 3")
+    _check(transforms.size(1) == 4, "transforms.size(1) == 4")
+    _check(transforms.size(2) == 4, "transforms.size(2) == 4")
+    _check(masks.size(0) == transforms.size(0), "masks.size(0) == transforms.size(0)")
+    _check(masks.size(3) == 1, "masks.size(3) == 1")
+    _check(int(level) >= 0, "level >= 0")
+    _check(float(cube_length) > 0.0, "cube_length > 0.f")
+
+
+def _scene_struct(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_corner_bfl: Sequence[float],
+                  cube_length: float, masks_partial: bool, transforms_convention: str,
+                  keep: list) -> _lib.Scene:
+    _require_gpu(masks)
+    _check_vh_args(masks, transforms, level, cube_length)
+    opengl = _convention(transforms_convention)
+    m, code = _masks_for_abi(masks)
+    T = transforms.to(torch.float32).contiguous()  # fmaf arguments are float (visual_hull_cuda.cu:150-153)
+    keep.extend([m, T])
+    corner = [float(c) for c in cube_corner_bfl]
+    _check(len(corner) == 3, "len(cube_corner_bfl) == 3")
+    s = _lib.Scene()
+    s.masks = m.data_ptr()
+    s.mask_dtype = code
+    s.transforms = T.data_ptr()
+    s.B, s.H, s.W = int(masks.size(0)), int(masks.size(1)), int(masks.size(2))
+    s.level = int(level)
+    s.masks_partial = int(bool(masks_partial))
+    s.opengl = opengl
+    s.cube_corner_bfl[0], s.cube_corner_bfl[1], s.cube_corner_bfl[2] = corner
+    s.cube_length = float(cube_length)
+    return s
+
+
+class _Run:
+    """One visual-hull batch executed by the library; results stay in the context until the next call."""
+
+    def __init__(self, scenes: list[_lib.Scene], device: torch.device, unique_verts: bool) -> None:
+        self.device = device
+        self.ctx = _lib.context(device.index if device.index is not None else torch.cuda.current_device())
+        self.stream = _stream(device)
+        arr = (_lib.Scene * len(scenes))(*scenes)
+        self.counts = (_lib.VhCounts * len(scenes))()
+        with torch.cuda.device(device):
+            _lib.check(_lib.lib().thull_visual_hull_run(self.ctx, arr, len(scenes), int(bool(unique_verts)),
+                                                        self.stream, self.counts))
+
+    def mesh(self, i: int) -> tuple[torch.Tensor, torch.Tensor]:
+        c = self.counts[i]
+        verts = torch.empty((c.num_verts, 3), dtype=torch.float32, device=self.device)
+        faces = torch.empty((c.num_faces, 3), dtype=torch.int64, device=self.device)
+        _lib.check(_lib.lib().thull_visual_hull_fetch_mesh(self.ctx, i, _ptr(verts), _ptr(faces), self.stream))
+        return verts, faces
+
+    def field(self, i: int, level: int) -> torch.Tensor:
+        c = self.counts[i]
+        n = c.num_field
+        idx = torch.empty((4, n), dtype=torch.int64, device=self.device)
+        vals = torch.empty((n,), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().thull_visual_hull_fetch_field(self.ctx, i, _ptr(idx), _ptr(vals), self.stream))
+        r = (1 << level) + 1
+        # to_sparse_coo_tensor (visual_hull_cuda.cu:575-577); indices are sorted and unique
+        return torch.sparse_coo_tensor(idx, vals, (1, r, r, r)).coalesce()
+
+    def candidates(self) -> list[torch.Tensor]:
+        L = _lib.lib()
+        out = []
+        for level in range(self.counts[0].num_levels):
+            n = ctypes.c_int64()
+            _lib.check(L.thull_visual_hull_candidates_count(self.ctx, level, ctypes.byref(n)))
+            t = torch.empty((n.value,), dtype=torch.int64, device=self.device)
+            _lib.check(L.thull_visual_hull_fetch_candidates(self.ctx, level, _ptr(t), self.stream))
+            out.append(t)
+        return out
+
+
+def visual_hull(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_corner_bfl: Sequence[float],
+                cube_length: float, masks_partial: bool, transforms_convention: str,
+                unique_verts: bool = True) -> tuple[torch.Tensor, torch.Tensor]:
+    """Visual hull of the masks as a mesh (bindings.cpp:12-57, visual_hull.cpp:34-58).
+
+    masks [B,H,W,1] (values in {0,1} or fractional), transforms [B,4,4]
+    (intrinsics * extrinsics), cube [corner, corner + length]^3 at resolution
+    2^level.  Returns verts float32 [V,3] and faces int64 [F,3]."""
+    keep: list = []
+    s = _scene_struct(masks, transforms, level, cube_corner_bfl, cube_length, masks_partial, transforms_convention,
+                      keep)
+    run = _Run([s], masks.device, unique_verts)
+    return run.mesh(0)
+
+
+def visual_hull_with_candidate_voxels(masks: torch.Tensor, transforms: torch.Tensor, level: int,
+                                      cube_corner_bfl: Sequence[float], cube_length: float, masks_partial: bool,
+                                      transforms_convention: str, unique_verts: bool = True
+                                      ) -> tuple[torch.Tensor, torch.Tensor, list[torch.Tensor]]:
+    """As visual_hull, plus the ravelled candidate cells of every level (bindings.cpp:59-105)."""
+    keep: list = []
+    s = _scene_struct(masks, transforms, level, cube_corner_bfl, cube_length, masks_partial, transforms_convention,
+                      keep)
+    run = _Run([s], masks.device, unique_verts)
+    verts, faces = run.mesh(0)
+    return verts, faces, run.candidates()
+
+
+def sparse_visual_hull_field(masks: torch.Tensor, transforms: torch.Tensor, level: int,
+                             cube_corner_bfl: Sequence[float], cube_length: float, masks_partial: bool,
+                             transforms_convention: str) -> torch.Tensor:
+    """Sparse field [1,R+1,R+1,R+1] (indices z, y, x) whose 0.5 isosurface is the hull (bindings.cpp:137-178)."""
+    keep: list = []
+    s = _scene_struct(masks, transforms, level, cube_corner_bfl, cube_length, masks_partial, transforms_convention,
+                      keep)
+    run = _Run([s], masks.device, True)
+    return run.field(0, int(level))
+
+
+def visual_hull_batch(scenes: Sequence[dict], unique_verts: bool = True) -> list[tuple[torch.Tensor, torch.Tensor]]:
+    """Visual hulls of many independent scenes in one batched pipeline run.
+
+    Each scene is a dict with the keyword arguments of visual_hull (masks,
+    transforms, level, cube_corner_bfl, cube_length, masks_partial,
+    transforms_convention).  Every level of every scene is carved in one
+    launch per octree level; results equal those of separate visual_hull calls."""
+    _check(len(scenes) > 0, "len(scenes) > 0")
+    keep: list = []
+    structs = []
+    device = scenes[0]["masks"].device
+    for sc in scenes:
+        _check(sc["masks"].device == device, "all scenes on one device")
+        structs.append(_scene_struct(sc["masks"], sc["transforms"], sc["level"], sc["cube_corner_bfl"],
+                                     sc["cube_length"], sc["masks_partial"], sc["transforms_convention"], keep))
+    run = _Run(structs, device, unique_verts)
+    return [run.mesh(i) for i in range(len(structs))]
+
+
+def candidate_voxels_to_wireframes(candidate_voxels: Sequence[torch.Tensor], cube_corner_bfl: Sequence[float],
+                                   cube_length: float) -> list[tuple[torch.Tensor, torch.Tensor]]:
+    """Wireframes of the candidate cells of every level (bindings.cpp:107-135, visual_hull_cuda.cu:1022-1111).
+
+    Level i cells are ravelled at resolution 2^i; each cell gives 8 vertices
+    fmaf(length / 2^i, corner_index, cube_corner) and 12 edges."""
+    _check(len(candidate_voxels) > 0, "candidate_voxels.empty() == false")
+    for c in candidate_voxels:
+        _check(c.device == candidate_voxels[0].device, "candidates.device() == candidate_voxels[0].device()")
+    _require_gpu(candidate_voxels[0])
+    device = candidate_voxels[0].device
+    corner = torch.tensor([float(v) for v in cube_corner_bfl], dtype=torch.float32, device=device)
+    # cube corner offsets (x, y, z) in cube_vertex order and the 12 edges in edge_to_vertex_table order
+    offs = torch.tensor([[i & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], dtype=torch.int64, device=device)
+    edges = torch.tensor([[0, 1], [1, 5], [4, 5], [0, 4], [2, 3], [3, 7], [6, 7], [2, 6], [0, 2], [1, 3], [5, 7],
+                          [4, 6]], dtype=torch.int64, device=device)
+    out = []
+    for i, cand in enumerate(candidate_voxels):
+        R = 1 << i
+        n = cand.numel()
+        g = torch.stack([cand % R, cand // R % R, cand // (R * R)], dim=1)
+        v = (g[:, None, :] + offs[None, :, :]).reshape(-1, 3).to(torch.float32)
+        scale = torch.tensor(float(cube_length), dtype=torch.float32) / torch.tensor(float(R), dtype=torch.float32)
+        verts = torch.addcmul(corner[None, :], v, scale.to(device))  # fmaf(scale, v, corner)
+        e = (edges[None, :, :] + 8 * torch.arange(n, device=device)[:, None, None]).reshape(-1, 2)
+        out.append((verts.reshape(8 * n, 3), e))
+    return out
+
+
+def marching_cubes(volume: torch.Tensor, isolevel: float, return_local_coords: bool = True,
+                   unique_verts: bool = True) -> tuple[torch.Tensor, torch.Tensor]:
+    """Marching cubes on a dense [1,D,H,W] or sparse COO volume (bindings.cpp:180-214, marching_cubes_cuda.cu:817-831)."""
+    _require_gpu(volume)
+    _check(volume.size(0) == 1, "volume.size(0) == 1")
+    _check(float(isolevel) >= 0, "isolevel >= 0")
+    device = volume.device
+    ctx = _lib.context(device.index if device.index is not None else torch.cuda.current_device())
+    st = _stream(device)
+    L = _lib.lib()
+    nv, nf = ctypes.c_int64(), ctypes.c_int64()
+    with torch.cuda.device(device):
+        if volume.is_sparse:
+            _check(volume.sparse_dim() == 4, "volume.sparse_dim() == 4")
+            _check(volume.dense_dim() == 0, "volume.dense_dim() == 0")
+            _check(volume.values().numel() < (1 << 31), "volume.values().numel() < 2^31")
+            v = volume if volume.is_coalesced() else volume.coalesce()
+            idx = v.indices().contiguous()
+            vals = v.values()
+            if vals.dtype not in (torch.float32, torch.float64):
+                vals = vals.to(torch.float32)
+            vals = vals.contiguous()
+            D, H, W = (int(s) for s in volume.shape[1:])
+            if idx.numel() > 0:
+                lo = idx.min(dim=1).values.tolist()
+                hi = idx.max(dim=1).values.tolist()
+                _check(min(lo) >= 0 and hi[1] < D and hi[2] < H and hi[3] < W, "indices within the volume")
+            _lib.check(L.thull_marching_cubes_sparse_run(ctx, _ptr(idx), _ptr(vals), _DTYPE_CODES[vals.dtype],
+                                                         vals.numel(), D, H, W, float(isolevel),
+                                                         int(bool(return_local_coords)), int(bool(unique_verts)), st,
+                                                         ctypes.byref(nv), ctypes.byref(nf)))
+        else:
+            _check(volume.dim() == 4, "volume.dim() == 4")
+            vol = volume if volume.dtype in (torch.float32, torch.float64) else volume.to(torch.float32)
+            vol = vol.contiguous()
+            D, H, W = (int(s) for s in volume.shape[1:])
+            _lib.check(L.thull_marching_cubes_dense_run(ctx, _ptr(vol), _DTYPE_CODES[vol.dtype], D, H, W,
+                                                        float(isolevel), int(bool(return_local_coords)),
+                                                        int(bool(unique_verts)), st, ctypes.byref(nv),
+                                                        ctypes.byref(nf)))
+        verts = torch.empty((nv.value, 3), dtype=torch.float32, device=device)
+        faces = torch.empty((nf.value, 3), dtype=torch.int64, device=device)
+        _lib.check(L.thull_marching_cubes_fetch(ctx, _ptr(verts), _ptr(faces), st))
+    return verts, faces
+
+
+def gaussian_blur(images: torch.Tensor, kernel_size: int, sigma: float, sparse: bool = True,
+                  dtype: torch.dtype | None = None) -> torch.Tensor:
+    """Gaussian blur of [B,H,W,C] images (bindings.cpp:216-251); the band blur (sparse=True) touches only
+    pixels near mask edges of channel 0, every other value is the input converted to `dtype`."""
+    _require_gpu(images)
+    _check(int(kernel_size) > 0, "kernel_size > 0")
+    _check(int(kernel_size) % 2 == 1, "kernel_size % 2 == 1")
+    _check(float(sigma) > 0.0, "sigma > 0.f")
+    _check(images.dim() == 4, "images.dim() == 4")
+    out_dtype = images.dtype if dtype is None else dtype
+    _check(out_dtype in (torch.float16, torch.float32, torch.float64), "isFloatingType(dtype)")
+    if images.dtype not in _DTYPE_CODES:
+        raise RuntimeError(f"gaussian_blur: unsupported input dtype {images.dtype}")
+    device = images.device
+    B, H, W, C = (int(s) for s in images.shape)
+    src = images[..., 0].contiguous()
+    ctx = _lib.context(device.index if device.index is not None else torch.cuda.current_device())
+    st = _stream(device)
+    with torch.cuda.device(device):
+        out0 = torch.empty((B, H, W), dtype=out_dtype, device=device)
+        _lib.check(_lib.lib().thull_gaussian_blur(ctx, _ptr(src), _DTYPE_CODES[images.dtype], B, H, W,
+                                                  int(kernel_size), float(sigma), int(bool(sparse)), _ptr(out0),
+                                                  _DTYPE_CODES[out_dtype], st))
+        if C == 1:
+            return out0.unsqueeze(-1)
+        if not sparse and int(kernel_size) > 1:
+            # the dense reference convolves every channel with the same kernel (gaussian_blur_cuda.cu:94-109)
+            chans = [out0]
+            for c in range(1, C):
+                sc = images[..., c].contiguous()
+                oc = torch.empty((B, H, W), dtype=out_dtype, device=device)
+                _lib.check(_lib.lib().thull_gaussian_blur(ctx, _ptr(sc), _DTYPE_CODES[images.dtype], B, H, W,
+                                                          int(kernel_size), float(sigma), 0, _ptr(oc),
+                                                          _DTYPE_CODES[out_dtype], st))
+                chans.append(oc)
+            return torch.stack(chans, dim=-1)
+        # sparse: only channel 0 is blurred (gaussian_blur_cuda.cu:221,230); the rest is the converted input
+        out = images.to(out_dtype).clone()
+        out[..., 0] = out0
+        return out
+
+
+def store_curve_network(filename: str, curve_network: tuple[torch.Tensor, torch.Tensor],
+                        verbose: bool = False) -> None:
+    """Writes a curve network (vertices [V,3], edges [E,2]) as OBJ or binary little-endian PLY (io.cpp:11-127)."""
+    path = pathlib.Path(filename)
+    if verbose:
+        print(f'Storing file "{filename}" ...')
+    t0 = time.perf_counter()
+    verts, edges = curve_network
+    v = verts.detach().to("cpu", torch.float32).contiguous()
+    e = edges.detach().to("cpu", torch.int64).contiguous()
+    suffix = path.suffix.lower()
+    if suffix == ".obj":
+        with open(path, "w") as f:
+            for x, y, z in v.tolist():
+                f.write(f"v {x:g} {y:g} {z:g}\n")
+            for a, b in e[:, :2].tolist():
+                f.write(f"l {a + 1} {b + 1}\n")  # 1-indexed
+    elif suffix == ".ply":
+        import numpy as np
+
+        header = ("ply\nformat binary_little_endian 1.0\n"
+                  f"element vertex {v.shape[0]}\nproperty float x\nproperty float y\nproperty float z\n"
+                  f"element edge {e.shape[0]}\nproperty int vertex1\nproperty int vertex2\nend_header\n")
+        with open(path, "wb") as f:
+            f.write(header.encode("ascii"))
+            f.write(v.numpy().astype("<f4").tobytes())
+            f.write(np.ascontiguousarray(e[:, :2].numpy().astype("<i4")).tobytes())
+    else:
+        raise RuntimeError(f'Unsupported file extension "{path.suffix}".')
+    if verbose:
+        print(f'Storing file "{filename}" ... done ({time.perf_counter() - t0:.3f}s)')

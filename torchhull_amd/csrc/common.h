@@ -1,0 +1,186 @@
+// Shared definitions of the MI355X visual-hull library: device-side scalar
+// semantics, scene descriptors, error handling and launch helpers.
+//
+// Floating-point rule of this library: every translation unit is compiled
+// with -ffp-contract=off, so the only fused multiply-adds are the explicit
+// fmaf() calls below, placed where the reference calls fmaf or where nvcc
+// contracts `a + b * c` by default.  Division is IEEE (hipcc's default
+// correctly rounded f32 divide), roundf rounds halves away from zero and
+// fminf/fmaxf ignore NaN operands, as in CUDA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace thull
+{
+
+// ---------------------------------------------------------------------------
+// Error handling
+// ---------------------------------------------------------------------------
+
+struct Error : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+#define THULL_CHECK(cond, msg)                                                                                         \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(cond))                                                                                                   \
+            throw ::thull::Error(std::string("Check failed: ") + #cond + ". " + (msg));                               \
+    } while (0)
+
+#define THULL_HIP(expr)                                                                                                \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t err_ = (expr);                                                                                      \
+        if (err_ != hipSuccess)                                                                                        \
+            throw ::thull::Error(std::string("HIP error ") + hipGetErrorString(err_) + " at " + __FILE__ + ":" +       \
+                                 std::to_string(__LINE__) + " (" #expr ")");                                           \
+    } while (0)
+
+#define THULL_LAUNCH_CHECK() THULL_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Element types crossing the C ABI (values of thull_dtype in the public header)
+// ---------------------------------------------------------------------------
+
+enum DType : int32_t
+{
+    kBool = 0,
+    kU8 = 1,
+    kI8 = 2,
+    kI16 = 3,
+    kI32 = 4,
+    kI64 = 5,
+    kF16 = 6,
+    kF32 = 7,
+    kF64 = 8,
+};
+
+// ---------------------------------------------------------------------------
+// Packed cell / vertex keys.  A grid point of scene s is packed as
+//   s << 48 | z << 32 | y << 16 | x     (x, y, z < 2^16, s < 2^16)
+// so that the octree queues and the vertex list carry their scene and
+// coordinates without any division.  Within one scene the packed order is the
+// (z, y, x) lexicographic order, i.e. the order of the reference's ravel
+// index x + y*R + z*R*R.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ uint64_t
+pack_key(uint32_t s, uint32_t x, uint32_t y, uint32_t z)
+{
+    return (uint64_t(s) << 48) | (uint64_t(z) << 32) | (uint64_t(y) << 16) | uint64_t(x);
+}
+__host__ __device__ __forceinline__ uint32_t
+key_scene(uint64_t k)
+{
+    return uint32_t(k >> 48);
+}
+__host__ __device__ __forceinline__ uint32_t
+key_x(uint64_t k)
+{
+    return uint32_t(k) & 0xffffu;
+}
+__host__ __device__ __forceinline__ uint32_t
+key_y(uint64_t k)
+{
+    return uint32_t(k >> 16) & 0xffffu;
+}
+__host__ __device__ __forceinline__ uint32_t
+key_z(uint64_t k)
+{
+    return uint32_t(k >> 32) & 0xffffu;
+}
+
+// ---------------------------------------------------------------------------
+// Device scalar semantics of the reference
+// ---------------------------------------------------------------------------
+
+// float -> int64 as CUDA's cvt.rzi.s64.f32 (static_cast<int64_t>(float)):
+// truncation, saturation to the int64 range, NaN -> 0.
+__device__ __forceinline__ int64_t
+f2i64_sat(float f)
+{
+    if (fabsf(f) < 2147483648.0f)
+        return int64_t(int32_t(f));
+    if (f != f)
+        return 0;
+    if (f >= 9223372036854775808.0f)
+        return INT64_MAX;
+    if (f < -9223372036854775808.0f)
+        return INT64_MIN;
+    return int64_t(f);
+}
+
+// clamp(int64(roundf-ed value r), 0, hi) evaluated in float: r is an
+// integral float, +-inf or NaN; NaN maps to 0 like the saturating conversion.
+__device__ __forceinline__ int32_t
+clamp_rounded(float r, float hi)
+{
+    return int32_t(fminf(fmaxf(r, 0.0f), hi));
+}
+
+// image_utils.h:67-71: ((c + 1) * size) / 2 - 0.5, every step rounded.
+__device__ __forceinline__ float
+unnormalize_ndc(float c, float size)
+{
+    float t = c + 1.0f;
+    t = t * size;
+    t = t * 0.5f; // exact, equal to t / 2
+    return t - 0.5f;
+}
+
+// math.h:19-23
+__device__ __forceinline__ float
+lerp_ref(float v0, float v1, float t)
+{
+    return fmaf(t, v1, fmaf(-t, v0, v0));
+}
+
+// Grid coordinate -> world coordinate: corner + v / res * length, contracted
+// by nvcc to fmaf(v / res, length, corner) (visual_hull_cuda.cu:206-213).
+// res is a power of two, so v / res is exact and equal to v * inv_res.
+__device__ __forceinline__ float
+grid_to_world(uint32_t v, float inv_res, float length, float corner)
+{
+    return fmaf(float(v) * inv_res, length, corner);
+}
+
+// ---------------------------------------------------------------------------
+// Scene descriptor (device side).  One per scene of a batch.
+// ---------------------------------------------------------------------------
+
+struct SceneDev
+{
+    const void* masks;      // [B,H,W] contiguous, element type mask_dtype
+    const float* integral;  // [B,H,W] float32 integral image
+    const float* transforms; // [B,16] float32
+    int32_t mask_dtype;
+    int32_t B, H, W;
+    int32_t level;          // target octree level
+    int32_t partial;        // masks_partial
+    int32_t opengl;         // transforms convention
+    float corner[3];
+    float length;
+    // bitmap of grid vertices (R+1)^3 with rows padded to `row_words` 32-bit words
+    uint64_t bitmap_offset; // in words, into the batch's bitmap
+    uint32_t row_words;
+    uint32_t pad_;
+};
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+
+inline unsigned
+blocks_for(int64_t n, int threads)
+{
+    return unsigned((n + threads - 1) / threads);
+}
+
+} // namespace thull

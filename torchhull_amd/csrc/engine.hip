@@ -1,0 +1,1035 @@
+// Host orchestration and the C ABI of the visual-hull library.
+//
+// The reference (vc-bonn/torchhull) orchestrates on the host in
+// src/torchhull/_C/src/visual_hull_cuda.cu:582-894 with torch allocations,
+// CUB/Thrust calls and a stream synchronisation after every launch.  Here one
+// batch of scenes goes through the whole pipeline on one stream; the host
+// waits only where a size decides the next allocation (one count per octree
+// level, the vertex count, the mesh sizes).
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <hip/hip_fp16.h>
+
+#include "../../include/torchhull_amd.h"
+#include "kernels.h"
+#include "scan.h"
+
+namespace thull
+{
+
+void launch_gaussian_blur_impl(const void* images, int32_t in_dtype, int64_t B, int64_t H, int64_t W, int k,
+                               bool sparse, const void* weights, void* out, int32_t out_dtype, uint8_t* flags,
+                               hipStream_t st);
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+// ---------------------------------------------------------------------------
+// Pooled device memory: chunks are kept for the context's lifetime and handed
+// out by a bump pointer that restarts with every call.
+// ---------------------------------------------------------------------------
+
+class Pool
+{
+public:
+    ~Pool() { release(); }
+
+    void
+    reset()
+    {
+        for (auto& c : chunks_)
+            c.used = 0;
+        in_use_ = 0;
+    }
+
+    template <typename T>
+    T*
+    alloc(uint64_t count)
+    {
+        return static_cast<T*>(alloc_bytes(count * sizeof(T)));
+    }
+
+    void*
+    alloc_bytes(uint64_t bytes)
+    {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        if (bytes == 0)
+            bytes = 256;
+        for (auto& c : chunks_)
+            if (c.size - c.used >= bytes)
+            {
+                void* p = c.ptr + c.used;
+                c.used += bytes;
+                note(bytes);
+                return p;
+            }
+        uint64_t reserved = 0;
+        for (auto& c : chunks_)
+            reserved += c.size;
+        const uint64_t size = std::max<uint64_t>(bytes, std::max<uint64_t>(uint64_t(64) << 20, reserved));
+        char* p = nullptr;
+        THULL_HIP(hipMalloc(&p, size));
+        chunks_.push_back(Chunk{ p, size, bytes });
+        note(bytes);
+        return p;
+    }
+
+    void
+    release()
+    {
+        for (auto& c : chunks_)
+            (void)hipFree(c.ptr);
+        chunks_.clear();
+    }
+
+    uint64_t
+    reserved() const
+    {
+        uint64_t r = 0;
+        for (auto& c : chunks_)
+            r += c.size;
+        return r;
+    }
+
+    uint64_t peak() const { return peak_; }
+
+private:
+    struct Chunk
+    {
+        char* ptr;
+        uint64_t size;
+        uint64_t used;
+    };
+
+    void
+    note(uint64_t bytes)
+    {
+        in_use_ += bytes;
+        peak_ = std::max(peak_, in_use_);
+    }
+
+    std::vector<Chunk> chunks_;
+    uint64_t in_use_ = 0;
+    uint64_t peak_ = 0;
+};
+
+// Pinned host staging for the few counts the host must see.
+struct HostStage
+{
+    unsigned long long* p = nullptr;
+    static constexpr int kSlots = 4096;
+    HostStage() { THULL_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), kSlots * sizeof(unsigned long long))); }
+    ~HostStage() { (void)hipHostFree(p); }
+};
+
+} // namespace
+
+struct Context
+{
+    int device = 0;
+    Pool pool;
+    HostStage* stage = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool have_last = false;
+
+    // last visual-hull run
+    int32_t n_scenes = 0;
+    bool unique = true;
+    std::vector<thull_vh_counts> counts;
+    std::vector<uint32_t> scene_vstart, scene_tstart, scene_first;
+    std::vector<SceneDev> scenes_host;
+    std::vector<uint64_t*> level_lists;
+    std::vector<uint32_t> level_sizes;
+    std::vector<int64_t> level_tests, level_children;
+    uint64_t* vertices = nullptr;
+    float* values = nullptr;
+    float* verts = nullptr;
+    int64_t* faces = nullptr;
+
+    // last marching-cubes run
+    float* mc_verts = nullptr;
+    int64_t* mc_faces = nullptr;
+    int64_t mc_nv = 0, mc_nf = 0;
+
+    void
+    begin(hipStream_t st)
+    {
+        THULL_HIP(hipSetDevice(device));
+        if (have_last && last_stream != st)
+            THULL_HIP(hipStreamSynchronize(last_stream));
+        last_stream = st;
+        have_last = true;
+        pool.reset();
+        level_lists.clear();
+        level_sizes.clear();
+    }
+
+    unsigned long long
+    read_u64(const unsigned long long* dev, hipStream_t st)
+    {
+        THULL_HIP(hipMemcpyAsync(stage->p, dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        THULL_HIP(hipStreamSynchronize(st));
+        return stage->p[0];
+    }
+};
+
+namespace
+{
+
+template <typename F>
+int
+guarded(F&& f)
+{
+    try
+    {
+        f();
+        g_last_error.clear();
+        return 0;
+    }
+    catch (const std::exception& e)
+    {
+        g_last_error = e.what();
+        return 1;
+    }
+    catch (...)
+    {
+        g_last_error = "unknown error";
+        return 1;
+    }
+}
+
+struct LoadPopcount
+{
+    const uint32_t* w;
+    __device__ uint32_t operator()(uint64_t i) const { return __popc(w[i]); }
+};
+
+struct LoadPopcountMask
+{
+    const uint32_t* m;
+    __device__ uint32_t operator()(uint64_t i) const { return __popc(m[i] & 7u); }
+};
+
+// Per scene: first vertex position, first output vertex and first triangle-vertex.
+__global__ void
+scene_starts_kernel(const uint64_t* word_begin,
+                    int32_t n,
+                    uint64_t nwords,
+                    const uint32_t* bitmap_prefix,
+                    uint64_t nverts,
+                    const uint32_t* vbase,
+                    const uint32_t* tbase,
+                    uint64_t vtotal,
+                    uint64_t ttotal,
+                    unsigned long long* out)
+{
+    const int32_t s = blockIdx.x * 256 + threadIdx.x;
+    if (s > n)
+        return;
+    uint64_t first = nverts;
+    if (s < n && word_begin[s] < nwords)
+        first = bitmap_prefix[word_begin[s]];
+    uint64_t vs = vtotal, ts = ttotal;
+    if (first < nverts)
+    {
+        vs = vbase != nullptr ? vbase[first] : 0;
+        ts = tbase[first];
+    }
+    out[3 * s + 0] = first;
+    out[3 * s + 1] = vs;
+    out[3 * s + 2] = ts;
+}
+
+__global__ void
+keys_to_ravel_kernel(const uint64_t* keys, uint32_t n, int32_t level, int64_t* out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t k = keys[i];
+    const int64_t R = int64_t(1) << level;
+    out[i] = int64_t(key_x(k)) + int64_t(key_y(k)) * R + int64_t(key_z(k)) * R * R;
+}
+
+__global__ void
+field_coo_kernel(const uint64_t* keys, const float* values, uint32_t n, int64_t* coo, float* vals)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t k = keys[i];
+    coo[i] = 0;
+    coo[n + i] = key_z(k);
+    coo[2 * uint64_t(n) + i] = key_y(k);
+    coo[3 * uint64_t(n) + i] = key_x(k);
+    vals[i] = values[i];
+}
+
+void
+check_u32(uint64_t v, const char* what)
+{
+    if (v >= (uint64_t(1) << 32))
+        throw Error(std::string(what) + " exceeds 2^32 entries");
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Visual hull
+// ---------------------------------------------------------------------------
+
+static void
+visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hipStream_t st, thull_vh_counts* counts)
+{
+    THULL_CHECK(n >= 1 && n < 65536, "n_scenes must be in [1, 65535]");
+    ctx.begin(st);
+    Pool& pool = ctx.pool;
+    ctx.n_scenes = n;
+    ctx.unique = unique;
+    ctx.counts.assign(n, thull_vh_counts{});
+    ctx.level_tests.clear();
+    ctx.level_children.clear();
+    ctx.vertices = nullptr;
+    ctx.values = nullptr;
+    ctx.verts = nullptr;
+    ctx.faces = nullptr;
+    std::vector<SceneDev> sd(n);
+    int32_t max_level = 0;
+    uint64_t total_words = 0;
+    std::vector<uint64_t> word_begin(n + 1, 0);
+    for (int32_t s = 0; s < n; ++s)
+    {
+        const thull_scene& sc = in[s];
+        THULL_CHECK(sc.level >= 0 && sc.level <= 15, "level must be in [0, 15]");
+        THULL_CHECK(sc.cube_length > 0.0f, "cube_length must be positive");
+        THULL_CHECK(sc.B >= 0 && sc.H >= 0 && sc.W >= 0, "mask sizes must be non-negative");
+        THULL_CHECK(sc.H * sc.W < (int64_t(1) << 31), "H * W must be below 2^31");
+        THULL_CHECK(sc.mask_dtype == kF32 || sc.mask_dtype == kU8 || sc.mask_dtype == kBool,
+                    "masks must be float32, uint8 or bool at the C ABI");
+        SceneDev d{};
+        d.masks = sc.masks;
+        d.mask_dtype = sc.mask_dtype;
+        d.transforms = sc.transforms;
+        d.B = int32_t(sc.B);
+        d.H = int32_t(sc.H);
+        d.W = int32_t(sc.W);
+        // a scene without cameras or pixels carves to nothing (every child is empty or undecided)
+        d.level = (sc.B == 0 || sc.H == 0 || sc.W == 0) ? -1 : sc.level;
+        d.partial = sc.masks_partial;
+        d.opengl = sc.opengl;
+        d.corner[0] = sc.cube_corner_bfl[0];
+        d.corner[1] = sc.cube_corner_bfl[1];
+        d.corner[2] = sc.cube_corner_bfl[2];
+        d.length = sc.cube_length;
+        const uint64_t R1 = (uint64_t(1) << sc.level) + 1;
+        d.row_words = uint32_t((R1 + 31) / 32);
+        d.bitmap_offset = total_words;
+        word_begin[s] = total_words;
+        if (d.level >= 0)
+            total_words += R1 * R1 * d.row_words;
+        max_level = std::max(max_level, d.level);
+        // integral image
+        if (d.level >= 0)
+        {
+            float* integral = pool.alloc<float>(uint64_t(sc.B) * sc.H * sc.W);
+            launch_integral_image(sc.masks, sc.mask_dtype, sc.B, sc.H, sc.W, integral, st);
+            d.integral = integral;
+        }
+        sd[s] = d;
+    }
+    word_begin[n] = total_words;
+    ctx.scenes_host = sd;
+    SceneDev* scenes = pool.alloc<SceneDev>(n);
+    THULL_HIP(hipMemcpyAsync(scenes, sd.data(), n * sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    unsigned long long* dev_total = pool.alloc<unsigned long long>(2);
+    unsigned long long* dev_tests = pool.alloc<unsigned long long>(1);
+
+    // ---- level 0: one root cell per scene that carves at all ----
+    std::vector<int32_t> live;
+    for (int32_t s = 0; s < n; ++s)
+        if (sd[s].level >= 0)
+            live.push_back(s);
+    uint64_t* list = pool.alloc<uint64_t>(std::max<size_t>(live.size(), 1));
+    {
+        std::vector<uint64_t> roots(live.size());
+        for (size_t k = 0; k < live.size(); ++k)
+            roots[k] = pack_key(uint32_t(live[k]), 0, 0, 0);
+        if (!roots.empty())
+            THULL_HIP(hipMemcpyAsync(list, roots.data(), roots.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    }
+    uint32_t count = uint32_t(live.size());
+    ctx.level_lists.push_back(list);
+    ctx.level_sizes.push_back(count);
+
+    // ---- level loop (VH:634-751) ----
+    for (int32_t i = 0; i < max_level && count > 0; ++i)
+    {
+        const uint32_t nb = blocks_for(count, kCarveThreads);
+        uint8_t* mask = pool.alloc<uint8_t>(count);
+        uint32_t* bcount = pool.alloc<uint32_t>(nb);
+        uint32_t* boff = pool.alloc<uint32_t>(nb);
+        uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nb) + 1);
+        THULL_HIP(hipMemsetAsync(dev_tests, 0, sizeof(unsigned long long), st));
+        launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests, st);
+        exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st);
+        THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        THULL_HIP(hipMemcpyAsync(ctx.stage->p + 1, dev_tests, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        THULL_HIP(hipStreamSynchronize(st));
+        const uint64_t next = ctx.stage->p[0];
+        ctx.level_tests.push_back(int64_t(ctx.stage->p[1]));
+        ctx.level_children.push_back(int64_t(count) * 8);
+        check_u32(next, "octree level size");
+        uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(next, 1));
+        launch_carve_scatter(list, count, mask, boff, children, st);
+        list = children;
+        count = uint32_t(next);
+        ctx.level_lists.push_back(list);
+        ctx.level_sizes.push_back(count);
+    }
+    const int32_t nlists = int32_t(ctx.level_lists.size());
+
+    // ---- corner vertices (VH:770-825) ----
+    uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
+    uint32_t* prefix = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
+    if (total_words > 0)
+        THULL_HIP(hipMemsetAsync(bitmap, 0, total_words * sizeof(uint32_t), st));
+    for (int32_t l = 0; l < nlists; ++l)
+    {
+        bool target = false;
+        for (int32_t s = 0; s < n; ++s)
+            target |= sd[s].level == l;
+        if (target)
+            launch_mark_corners(ctx.level_lists[l], ctx.level_sizes[l], l, scenes, bitmap, st);
+    }
+    uint32_t* wsums = pool.alloc<uint32_t>(scan_blocks(total_words) + 1);
+    exclusive_scan(LoadPopcount{ bitmap }, total_words, prefix, wsums, dev_total, st);
+    const uint64_t nverts = ctx.read_u64(dev_total, st);
+    check_u32(nverts, "sparse field size");
+    uint64_t* dev_word_begin = pool.alloc<uint64_t>(n + 1);
+    THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    uint64_t* vertices = pool.alloc<uint64_t>(std::max<uint64_t>(nverts, 1));
+    float* values = pool.alloc<float>(std::max<uint64_t>(nverts, 1));
+    launch_extract_vertices(bitmap, prefix, total_words, dev_word_begin, scenes, n, vertices, st);
+
+    // ---- field (VH:827-888) ----
+    launch_field(vertices, uint32_t(nverts), scenes, values, st);
+    ctx.vertices = vertices;
+    ctx.values = values;
+
+    // ---- sparse marching cubes at isolevel 0.5, world coordinates, flipped faces ----
+    std::vector<McGrid> grids(n);
+    for (int32_t s = 0; s < n; ++s)
+    {
+        McGrid g{};
+        const uint32_t R1 = (1u << std::max(0, in[s].level)) + 1u;
+        g.dims[0] = g.dims[1] = g.dims[2] = R1;
+        g.row_words = sd[s].row_words;
+        g.bitmap_offset = sd[s].bitmap_offset;
+        g.out_mode = 2;
+        const float scale = in[s].cube_length / float(int64_t(1) << std::max(0, in[s].level));
+        g.scale[0] = g.scale[1] = g.scale[2] = scale;
+        g.corner[0] = in[s].cube_corner_bfl[0];
+        g.corner[1] = in[s].cube_corner_bfl[1];
+        g.corner[2] = in[s].cube_corner_bfl[2];
+        g.flip = 1;
+        grids[s] = g;
+    }
+    McGrid* dgrids = pool.alloc<McGrid>(n);
+    THULL_HIP(hipMemcpyAsync(dgrids, grids.data(), n * sizeof(McGrid), hipMemcpyHostToDevice, st));
+    McPlan plan{};
+    plan.grids = dgrids;
+    plan.vertices = vertices;
+    plan.nverts = nverts;
+    plan.values = values;
+    plan.value_type = kMcF32;
+    plan.iso = 0.5f;
+    plan.bitmap = bitmap;
+    plan.prefix = prefix;
+    plan.dense = false;
+
+    uint64_t vtotal = 0, ttotal = 0;
+    float* verts = nullptr;
+    int64_t* faces = nullptr;
+    std::vector<unsigned long long> starts(3 * (n + 1), 0);
+    if (nverts > 0)
+    {
+        uint32_t* info = pool.alloc<uint32_t>(nverts);
+        uint32_t* ntv = pool.alloc<uint32_t>(nverts);
+        uint32_t* tbase = pool.alloc<uint32_t>(nverts);
+        uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nverts) + 1);
+        uint32_t* emask = nullptr;
+        uint32_t* vbase = nullptr;
+        launch_mc_classify(plan, info, ntv, st);
+        exclusive_scan(LoadU32{ ntv }, nverts, tbase, sums, dev_total, st);
+        if (unique)
+        {
+            emask = pool.alloc<uint32_t>(nverts);
+            vbase = pool.alloc<uint32_t>(nverts);
+            launch_mc_edges(plan, info, emask, st);
+            exclusive_scan(LoadPopcountMask{ emask }, nverts, vbase, sums, dev_total + 1, st);
+        }
+        else
+            THULL_HIP(hipMemsetAsync(dev_total + 1, 0, sizeof(unsigned long long), st));
+        THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        THULL_HIP(hipStreamSynchronize(st));
+        ttotal = ctx.stage->p[0];
+        vtotal = unique ? ctx.stage->p[1] : ttotal;
+        check_u32(ttotal, "triangle vertex count");
+        unsigned long long* dstarts = pool.alloc<unsigned long long>(3 * (n + 1));
+        scene_starts_kernel<<<blocks_for(n + 1, 256), 256, 0, st>>>(dev_word_begin, n, total_words, prefix, nverts,
+                                                                    vbase, tbase, vtotal, ttotal, dstarts);
+        THULL_LAUNCH_CHECK();
+        THULL_HIP(hipMemcpyAsync(starts.data(), dstarts, starts.size() * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, st));
+        std::vector<uint32_t> vstart(n + 1), tstart(n + 1);
+        THULL_HIP(hipStreamSynchronize(st));
+        for (int32_t s = 0; s <= n; ++s)
+        {
+            vstart[s] = uint32_t(starts[3 * s + 1]);
+            tstart[s] = uint32_t(starts[3 * s + 2]);
+        }
+        uint32_t* dv = pool.alloc<uint32_t>(n + 1);
+        uint32_t* dt = pool.alloc<uint32_t>(n + 1);
+        THULL_HIP(hipMemcpyAsync(dv, vstart.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        THULL_HIP(hipMemcpyAsync(dt, tstart.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        verts = pool.alloc<float>(3 * std::max<uint64_t>(vtotal, 1));
+        faces = pool.alloc<int64_t>(std::max<uint64_t>(ttotal, 1));
+        if (unique)
+            launch_mc_vertices(plan, emask, vbase, verts, st);
+        launch_mc_faces(plan, info, tbase, emask, vbase, dv, dt, unique, verts, faces, st);
+        ctx.scene_vstart = vstart;
+        ctx.scene_tstart = tstart;
+        // pageable host vectors above were copied asynchronously: wait before they go out of scope
+        THULL_HIP(hipStreamSynchronize(st));
+    }
+    else
+    {
+        ctx.scene_vstart.assign(n + 1, 0);
+        ctx.scene_tstart.assign(n + 1, 0);
+    }
+    ctx.verts = verts;
+    ctx.faces = faces;
+    ctx.scene_first.assign(n + 1, 0);
+    for (int32_t s = 0; s <= n; ++s)
+        ctx.scene_first[s] = nverts > 0 ? uint32_t(starts[3 * s + 0]) : 0u;
+    for (int32_t s = 0; s < n; ++s)
+    {
+        thull_vh_counts c{};
+        c.num_verts = int64_t(ctx.scene_vstart[s + 1]) - int64_t(ctx.scene_vstart[s]);
+        c.num_faces = (int64_t(ctx.scene_tstart[s + 1]) - int64_t(ctx.scene_tstart[s])) / 3;
+        c.num_field = int64_t(ctx.scene_first[s + 1]) - int64_t(ctx.scene_first[s]);
+        // candidate lists kept for this scene: up to its level, or up to the first empty level
+        int32_t nl = 1;
+        if (sd[s].level >= 0)
+            nl = std::min(in[s].level + 1, nlists);
+        c.num_levels = nl;
+        counts[s] = c;
+    }
+    ctx.counts.assign(counts, counts + n);
+}
+
+} // namespace thull
+
+using thull::Context;
+using thull::guarded;
+
+struct thull_ctx
+{
+    Context c;
+};
+
+extern "C" {
+
+int
+thull_abi_version(void)
+{
+    return THULL_ABI_VERSION;
+}
+
+const char*
+thull_last_error(void)
+{
+    return thull::g_last_error.c_str();
+}
+
+int
+thull_ctx_create(int device, thull_ctx** out)
+{
+    return guarded([&] {
+        THULL_CHECK(out != nullptr, "out is null");
+        int n = 0;
+        THULL_HIP(hipGetDeviceCount(&n));
+        THULL_CHECK(device >= 0 && device < n, "invalid device index");
+        THULL_HIP(hipSetDevice(device));
+        auto* ctx = new thull_ctx{};
+        ctx->c.device = device;
+        try
+        {
+            ctx->c.stage = new thull::HostStage();
+        }
+        catch (...)
+        {
+            delete ctx;
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int
+thull_ctx_destroy(thull_ctx* ctx)
+{
+    return guarded([&] {
+        if (ctx == nullptr)
+            return;
+        (void)hipSetDevice(ctx->c.device);
+        (void)hipDeviceSynchronize();
+        delete ctx->c.stage;
+        delete ctx;
+    });
+}
+
+int
+thull_ctx_memory(thull_ctx* ctx, int64_t* peak, int64_t* reserved)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        if (peak)
+            *peak = int64_t(ctx->c.pool.peak());
+        if (reserved)
+            *reserved = int64_t(ctx->c.pool.reserved());
+    });
+}
+
+int
+thull_ctx_trim(thull_ctx* ctx)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        THULL_HIP(hipSetDevice(ctx->c.device));
+        THULL_HIP(hipDeviceSynchronize());
+        ctx->c.pool.release();
+        ctx->c.have_last = false;
+        ctx->c.verts = nullptr;
+        ctx->c.faces = nullptr;
+        ctx->c.vertices = nullptr;
+        ctx->c.values = nullptr;
+        ctx->c.level_lists.clear();
+        ctx->c.mc_verts = nullptr;
+        ctx->c.mc_faces = nullptr;
+    });
+}
+
+int
+thull_visual_hull_run(thull_ctx* ctx,
+                      const thull_scene* scenes,
+                      int32_t n_scenes,
+                      int32_t unique_verts,
+                      void* stream,
+                      thull_vh_counts* counts)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr && scenes != nullptr && counts != nullptr, "null argument");
+        thull::visual_hull_run(ctx->c, scenes, n_scenes, unique_verts != 0, static_cast<hipStream_t>(stream), counts);
+    });
+}
+
+int
+thull_visual_hull_fetch_mesh(thull_ctx* ctx, int32_t scene, float* verts, int64_t* faces, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        THULL_CHECK(scene >= 0 && scene < c.n_scenes, "scene index out of range");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const thull_vh_counts& k = c.counts[scene];
+        if (k.num_verts > 0)
+            THULL_HIP(hipMemcpyAsync(verts, c.verts + 3 * uint64_t(c.scene_vstart[scene]),
+                                     3 * sizeof(float) * uint64_t(k.num_verts), hipMemcpyDeviceToDevice, st));
+        if (k.num_faces > 0)
+            THULL_HIP(hipMemcpyAsync(faces, c.faces + uint64_t(c.scene_tstart[scene]),
+                                     3 * sizeof(int64_t) * uint64_t(k.num_faces), hipMemcpyDeviceToDevice, st));
+    });
+}
+
+int
+thull_visual_hull_fetch_field(thull_ctx* ctx, int32_t scene, int64_t* indices, float* values, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        THULL_CHECK(scene >= 0 && scene < c.n_scenes, "scene index out of range");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const uint32_t n = uint32_t(c.counts[scene].num_field);
+        if (n == 0)
+            return;
+        const uint32_t first = c.scene_first[scene];
+        thull::field_coo_kernel<<<thull::blocks_for(n, 256), 256, 0, st>>>(c.vertices + first, c.values + first, n,
+                                                                          indices, values);
+        THULL_LAUNCH_CHECK();
+    });
+}
+
+int
+thull_visual_hull_candidates_count(thull_ctx* ctx, int32_t level, int64_t* count)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr && count != nullptr, "null argument");
+        Context& c = ctx->c;
+        THULL_CHECK(c.n_scenes == 1, "candidate lists are kept for single-scene runs only");
+        THULL_CHECK(level >= 0 && level < c.counts[0].num_levels, "level out of range");
+        *count = int64_t(c.level_sizes[level]);
+    });
+}
+
+int
+thull_visual_hull_fetch_candidates(thull_ctx* ctx, int32_t level, int64_t* out, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        THULL_CHECK(c.n_scenes == 1, "candidate lists are kept for single-scene runs only");
+        THULL_CHECK(level >= 0 && level < c.counts[0].num_levels, "level out of range");
+        const uint32_t n = c.level_sizes[level];
+        if (n == 0)
+            return;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        thull::keys_to_ravel_kernel<<<thull::blocks_for(n, 256), 256, 0, st>>>(c.level_lists[level], n, level, out);
+        THULL_LAUNCH_CHECK();
+    });
+}
+
+int
+thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_tests, int64_t* children)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        for (int32_t i = 0; i < max_levels; ++i)
+        {
+            const bool have = i < int32_t(c.level_tests.size());
+            if (camera_tests)
+                camera_tests[i] = have ? c.level_tests[i] : 0;
+            if (children)
+                children[i] = have ? c.level_children[i] : 0;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Marching cubes API
+// ---------------------------------------------------------------------------
+
+static void
+mc_run_common(Context& c, thull::McPlan plan, thull::McGrid grid, bool unique, hipStream_t st, int64_t* nv,
+              int64_t* nf, uint32_t* bitmap_for_starts)
+{
+    (void)bitmap_for_starts;
+    using namespace thull;
+    Pool& pool = c.pool;
+    thull::McGrid* dgrid = pool.alloc<McGrid>(1);
+    THULL_HIP(hipMemcpyAsync(dgrid, &grid, sizeof(McGrid), hipMemcpyHostToDevice, st));
+    plan.grids = dgrid;
+    c.mc_verts = nullptr;
+    c.mc_faces = nullptr;
+    c.mc_nv = c.mc_nf = 0;
+    *nv = *nf = 0;
+    const uint64_t n = plan.nverts;
+    if (n == 0)
+    {
+        THULL_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    check_u32(n, "marching cubes vertex count");
+    unsigned long long* dev_total = pool.alloc<unsigned long long>(2);
+    uint32_t* info = pool.alloc<uint32_t>(n);
+    uint32_t* ntv = pool.alloc<uint32_t>(n);
+    uint32_t* tbase = pool.alloc<uint32_t>(n);
+    uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(n) + 1);
+    uint32_t* emask = nullptr;
+    uint32_t* vbase = nullptr;
+    launch_mc_classify(plan, info, ntv, st);
+    exclusive_scan(LoadU32{ ntv }, n, tbase, sums, dev_total, st);
+    if (unique)
+    {
+        emask = pool.alloc<uint32_t>(n);
+        vbase = pool.alloc<uint32_t>(n);
+        launch_mc_edges(plan, info, emask, st);
+        exclusive_scan(LoadPopcountMask{ emask }, n, vbase, sums, dev_total + 1, st);
+    }
+    else
+        THULL_HIP(hipMemsetAsync(dev_total + 1, 0, sizeof(unsigned long long), st));
+    THULL_HIP(hipMemcpyAsync(c.stage->p, dev_total, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    THULL_HIP(hipStreamSynchronize(st));
+    const uint64_t ttotal = c.stage->p[0];
+    const uint64_t vtotal = unique ? c.stage->p[1] : ttotal;
+    check_u32(ttotal, "triangle vertex count");
+    uint32_t* zeros = pool.alloc<uint32_t>(2);
+    THULL_HIP(hipMemsetAsync(zeros, 0, 2 * sizeof(uint32_t), st));
+    float* verts = pool.alloc<float>(3 * std::max<uint64_t>(vtotal, 1));
+    int64_t* faces = pool.alloc<int64_t>(std::max<uint64_t>(ttotal, 1));
+    if (unique)
+        launch_mc_vertices(plan, emask, vbase, verts, st);
+    launch_mc_faces(plan, info, tbase, emask, vbase, zeros, zeros, unique, verts, faces, st);
+    c.mc_verts = verts;
+    c.mc_faces = faces;
+    c.mc_nv = int64_t(vtotal);
+    c.mc_nf = int64_t(ttotal / 3);
+    *nv = c.mc_nv;
+    *nf = c.mc_nf;
+}
+
+static thull::McGrid
+mc_grid(int64_t D, int64_t H, int64_t W, bool local)
+{
+    thull::McGrid g{};
+    g.dims[0] = uint32_t(W);
+    g.dims[1] = uint32_t(H);
+    g.dims[2] = uint32_t(D);
+    g.row_words = uint32_t((W + 31) / 32);
+    g.bitmap_offset = 0;
+    g.out_mode = local ? 1 : 0;
+    // MC:523-526: 2 / (size - 1) per axis, x with W, y with H, z with D
+    g.scale[0] = 2.0f / float(W - 1);
+    g.scale[1] = 2.0f / float(H - 1);
+    g.scale[2] = 2.0f / float(D - 1);
+    g.flip = 0;
+    return g;
+}
+
+int
+thull_marching_cubes_dense_run(thull_ctx* ctx,
+                               const void* volume,
+                               int32_t dtype,
+                               int64_t D,
+                               int64_t H,
+                               int64_t W,
+                               float isolevel,
+                               int32_t return_local_coords,
+                               int32_t unique_verts,
+                               void* stream,
+                               int64_t* num_verts,
+                               int64_t* num_faces)
+{
+    return guarded([&] {
+        using namespace thull;
+        THULL_CHECK(ctx != nullptr && num_verts != nullptr && num_faces != nullptr, "null argument");
+        THULL_CHECK(dtype == kF32 || dtype == kF64, "volume must be float32 or float64 at the C ABI");
+        THULL_CHECK(isolevel >= 0.0f, "isolevel must be non-negative");
+        THULL_CHECK(D >= 0 && H >= 0 && W >= 0 && D < 65536 && H < 65536 && W < 65536, "volume size out of range");
+        THULL_CHECK(D * H * W < (int64_t(1) << 32), "volume must have fewer than 2^32 elements");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        c.begin(st);
+        McPlan plan{};
+        plan.vertices = nullptr;
+        plan.nverts = (D < 2 || H < 2 || W < 2) ? 0 : uint64_t(D * H * W);
+        plan.values = volume;
+        plan.value_type = dtype == kF64 ? kMcF64 : kMcF32;
+        plan.iso = isolevel;
+        plan.dense = true;
+        plan.dense_dims[0] = uint32_t(W);
+        plan.dense_dims[1] = uint32_t(H);
+        plan.dense_dims[2] = uint32_t(D);
+        mc_run_common(c, plan, mc_grid(D, H, W, return_local_coords != 0), unique_verts != 0, st, num_verts,
+                      num_faces, nullptr);
+    });
+}
+
+int
+thull_marching_cubes_sparse_run(thull_ctx* ctx,
+                                const int64_t* indices,
+                                const void* values,
+                                int32_t dtype,
+                                int64_t nnz,
+                                int64_t D,
+                                int64_t H,
+                                int64_t W,
+                                float isolevel,
+                                int32_t return_local_coords,
+                                int32_t unique_verts,
+                                void* stream,
+                                int64_t* num_verts,
+                                int64_t* num_faces)
+{
+    return guarded([&] {
+        using namespace thull;
+        THULL_CHECK(ctx != nullptr && num_verts != nullptr && num_faces != nullptr, "null argument");
+        THULL_CHECK(dtype == kF32 || dtype == kF64, "values must be float32 or float64 at the C ABI");
+        THULL_CHECK(isolevel >= 0.0f, "isolevel must be non-negative");
+        THULL_CHECK(nnz >= 0 && nnz < (int64_t(1) << 31), "nnz must be below 2^31");
+        THULL_CHECK(D >= 0 && H >= 0 && W >= 0 && D < 65536 && H < 65536 && W < 65536, "volume size out of range");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        c.begin(st);
+        McGrid grid = mc_grid(D, H, W, return_local_coords != 0);
+        const uint64_t nwords = uint64_t(D) * uint64_t(H) * grid.row_words;
+        THULL_CHECK(nwords < (uint64_t(1) << 34), "sparse volume too large for the vertex bitmap");
+        McPlan plan{};
+        plan.nverts = 0;
+        if (nnz > 0 && nwords > 0)
+        {
+            Pool& pool = c.pool;
+            McGrid* dgrid = pool.alloc<McGrid>(1);
+            THULL_HIP(hipMemcpyAsync(dgrid, &grid, sizeof(McGrid), hipMemcpyHostToDevice, st));
+            uint32_t* bitmap = pool.alloc<uint32_t>(nwords);
+            uint32_t* prefix = pool.alloc<uint32_t>(nwords);
+            THULL_HIP(hipMemsetAsync(bitmap, 0, nwords * sizeof(uint32_t), st));
+            launch_mark_coo(indices, uint64_t(nnz), dgrid, bitmap, st);
+            uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nwords) + 1);
+            unsigned long long* dev_total = pool.alloc<unsigned long long>(1);
+            exclusive_scan(LoadPopcount{ bitmap }, nwords, prefix, sums, dev_total, st);
+            const uint64_t nv = c.read_u64(dev_total, st);
+            THULL_CHECK(nv == uint64_t(nnz), "sparse volume must be coalesced (unique, in-range indices)");
+            uint64_t* vertices = pool.alloc<uint64_t>(nv);
+            launch_extract_grid_vertices(bitmap, prefix, nwords, dgrid, vertices, st);
+            plan.vertices = vertices;
+            plan.nverts = nv;
+            plan.bitmap = bitmap;
+            plan.prefix = prefix;
+        }
+        plan.values = values;
+        plan.value_type = dtype == kF64 ? kMcF64 : kMcF32;
+        plan.iso = isolevel;
+        plan.dense = false;
+        mc_run_common(c, plan, grid, unique_verts != 0, st, num_verts, num_faces, nullptr);
+    });
+}
+
+int
+thull_marching_cubes_fetch(thull_ctx* ctx, float* verts, int64_t* faces, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (c.mc_nv > 0)
+            THULL_HIP(hipMemcpyAsync(verts, c.mc_verts, 3 * sizeof(float) * uint64_t(c.mc_nv), hipMemcpyDeviceToDevice,
+                                     st));
+        if (c.mc_nf > 0)
+            THULL_HIP(hipMemcpyAsync(faces, c.mc_faces, 3 * sizeof(int64_t) * uint64_t(c.mc_nf),
+                                     hipMemcpyDeviceToDevice, st));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian blur and integral image
+// ---------------------------------------------------------------------------
+
+// Weight of tap (i, j) as evaluated by the reference (gaussian_blur_cuda.cu:222-223):
+// expf(-float(i*i + j*j) / (2 * sigma * sigma)) with float32 argument; the
+// exponential is taken correctly rounded (double exp, rounded to float).
+static float
+gauss_weight(int i, int j, float sigma)
+{
+    const float num = -float(i * i + j * j);
+    float den = 2.0f * sigma;
+    den = den * sigma;
+    const float arg = num / den;
+    return float(std::exp(double(arg)));
+}
+
+int
+thull_gaussian_blur(thull_ctx* ctx,
+                    const void* images,
+                    int32_t in_dtype,
+                    int64_t B,
+                    int64_t H,
+                    int64_t W,
+                    int32_t kernel_size,
+                    float sigma,
+                    int32_t sparse,
+                    void* out,
+                    int32_t out_dtype,
+                    void* stream)
+{
+    return guarded([&] {
+        using namespace thull;
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        THULL_CHECK(kernel_size > 0, "kernel_size must be positive");
+        THULL_CHECK(kernel_size % 2 == 1, "kernel_size must be odd");
+        THULL_CHECK(sigma > 0.0f, "sigma must be positive");
+        THULL_CHECK(out_dtype == kF16 || out_dtype == kF32 || out_dtype == kF64, "dtype must be floating point");
+        THULL_CHECK(B >= 0 && H >= 0 && W >= 0, "sizes must be non-negative");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        c.begin(st);
+        const int k = kernel_size, h = k / 2;
+        std::vector<float> w(size_t(k) * k);
+        for (int i = -h; i <= h; ++i)
+            for (int j = -h; j <= h; ++j)
+                w[(i + h) * k + (j + h)] = gauss_weight(i, j, sigma);
+        const bool is_sparse = sparse != 0;
+        std::vector<unsigned char> table;
+        if (out_dtype == kF32)
+        {
+            std::vector<float> t(w);
+            if (!is_sparse)
+            {
+                float sum = 0.0f; // GB:68: kernel / float sum
+                for (float v : w)
+                    sum += v;
+                for (auto& v : t)
+                    v = v / sum;
+            }
+            table.resize(t.size() * sizeof(float));
+            std::memcpy(table.data(), t.data(), table.size());
+        }
+        else if (out_dtype == kF64)
+        {
+            std::vector<double> t(w.size());
+            float sum = 0.0f;
+            for (float v : w)
+                sum += v;
+            for (size_t q = 0; q < w.size(); ++q)
+                t[q] = is_sparse ? double(w[q]) : double(w[q]) / double(sum);
+            table.resize(t.size() * sizeof(double));
+            std::memcpy(table.data(), t.data(), table.size());
+        }
+        else
+        {
+            std::vector<__half> t(w.size());
+            float sum = 0.0f;
+            for (float v : w)
+                sum += __half2float(__float2half(v));
+            for (size_t q = 0; q < w.size(); ++q)
+            {
+                const __half hw = __float2half(w[q]);
+                t[q] = is_sparse ? hw : __float2half(__half2float(hw) / sum);
+            }
+            table.resize(t.size() * sizeof(__half));
+            std::memcpy(table.data(), t.data(), table.size());
+        }
+        if (B == 0 || H == 0 || W == 0)
+            return;
+        void* dweights = c.pool.alloc_bytes(table.size());
+        THULL_HIP(hipMemcpyAsync(dweights, table.data(), table.size(), hipMemcpyHostToDevice, st));
+        uint8_t* flags = nullptr;
+        if (is_sparse && k > 1)
+            flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
+        launch_gaussian_blur_impl(images, in_dtype, B, H, W, k, is_sparse, dweights, out, out_dtype, flags, st);
+        // the weight table above is pageable host memory read by an async copy
+        THULL_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int
+thull_integral_image(thull_ctx* ctx, const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out,
+                     void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        THULL_CHECK(H * W < (int64_t(1) << 31), "H * W must be below 2^31");
+        THULL_HIP(hipSetDevice(ctx->c.device));
+        thull::launch_integral_image(masks, dtype, B, H, W, out, static_cast<hipStream_t>(stream));
+    });
+}
+
+} // extern "C"

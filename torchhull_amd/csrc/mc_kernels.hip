@@ -1,0 +1,508 @@
+// Marching cubes for CDNA4 over a sorted vertex set (sparse) or a dense grid.
+//
+// Reference (vc-bonn/torchhull, src/torchhull/_C/src/marching_cubes_cuda.cu, MC):
+// classification MC:146-223, generation MC:242-374, interpolation MC:72-97,
+// edge keys MC:99-112, unique vertices MC:674-693.
+//
+// The reference emits a triangle soup, keys every soup vertex by
+// (ravel(lower endpoint) << 3 | direction) and runs unique_dim (a full sort)
+// to build the compact mesh, whose vertices come out in ascending key order.
+// This implementation reaches the same mesh without sorting: an output vertex
+// is a grid edge (p, d) owned by its lower endpoint p.  An edge is emitted iff
+// its endpoints lie on different sides of the isolevel and at least one of the
+// (up to four) cells around it is complete; for a complete cell the case table
+// lists exactly its sign-changing edges, so this is the set of edges the
+// reference's soup contains.  The vertex list is sorted by ravel index, so an
+// exclusive scan of popcount(edge mask) over it numbers the edges in the
+// reference's ascending key order, and a face corner finds its vertex index
+// by a rank lookup of p.  Duplicate soup vertices of the reference are
+// bit-identical (same endpoints and values), so computing each edge's position
+// once gives the same coordinates.
+
+#include "mc_cases.h"
+#include "kernels.h"
+
+namespace thull
+{
+
+__constant__ McCaseTable c_cases = mc_build_case_table();
+
+// Case corner b -> offset (bit0 x, bit1 y, bit2 z) in the cell.
+__constant__ uint8_t c_corner_off[8] = { 0b000, 0b001, 0b101, 0b100, 0b010, 0b011, 0b111, 0b110 };
+// Edge e -> lower endpoint offset (bits as above) and axis (0 x, 1 y, 2 z).
+__constant__ uint8_t c_edge_low[12] = { 0b000, 0b001, 0b100, 0b000, 0b010, 0b011, 0b110, 0b010, 0b000, 0b001, 0b101, 0b100 };
+__constant__ uint8_t c_edge_axis[12] = { 0, 2, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1 };
+
+constexpr uint32_t kInfoComplete = 1u << 8;
+constexpr int kInfoSignShift = 9;
+
+struct BitmapLookup
+{
+    const uint32_t* bitmap;
+    const uint32_t* prefix;
+
+    __device__ __forceinline__ bool
+    find(const McGrid& g, uint32_t x, uint32_t y, uint32_t z, uint32_t& pos) const
+    {
+        const uint64_t w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
+        const uint32_t word = bitmap[w];
+        const uint32_t bit = x & 31u;
+        if (((word >> bit) & 1u) == 0u)
+            return false;
+        pos = prefix[w] + __popc(word & ((1u << bit) - 1u));
+        return true;
+    }
+};
+
+struct DenseLookup
+{
+    __device__ __forceinline__ bool
+    find(const McGrid& g, uint32_t x, uint32_t y, uint32_t z, uint32_t& pos) const
+    {
+        pos = x + g.dims[0] * (y + g.dims[1] * z);
+        return true;
+    }
+};
+
+struct VertexRef
+{
+    uint32_t scene, x, y, z;
+};
+
+__device__ __forceinline__ VertexRef
+vertex_ref(const McPlan& p, uint64_t i)
+{
+    if (p.dense)
+    {
+        const uint64_t sx = p.dense_dims[0], sy = p.dense_dims[1];
+        const uint64_t row = i / sx;
+        return VertexRef{ 0u, uint32_t(i - row * sx), uint32_t(row % sy), uint32_t(row / sy) };
+    }
+    const uint64_t k = p.vertices[i];
+    return VertexRef{ key_scene(k), key_x(k), key_y(k), key_z(k) };
+}
+
+template <typename V>
+__device__ __forceinline__ V
+value_at(const McPlan& p, uint32_t pos)
+{
+    return static_cast<const V*>(p.values)[pos];
+}
+
+// MC:72-97
+__device__ __forceinline__ void
+interpolate_vertex(float iso, const uint32_t p1[3], const uint32_t p2[3], float v1, float v2, float out[3])
+{
+    const float eps = 1e-5f;
+    if (fabsf(iso - v1) < eps)
+    {
+        out[0] = float(p1[0]);
+        out[1] = float(p1[1]);
+        out[2] = float(p1[2]);
+        return;
+    }
+    if (fabsf(iso - v2) < eps)
+    {
+        out[0] = float(p2[0]);
+        out[1] = float(p2[1]);
+        out[2] = float(p2[2]);
+        return;
+    }
+    if (fabsf(v1 - v2) < eps)
+    {
+        out[0] = float(p1[0]);
+        out[1] = float(p1[1]);
+        out[2] = float(p1[2]);
+        return;
+    }
+    const float t = (iso - v1) / (v2 - v1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        out[a] = lerp_ref(float(p1[a]), float(p2[a]), t);
+}
+
+__device__ __forceinline__ void
+finish_vertex(const McGrid& g, const float v[3], float* out)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        float r = v[a];
+        if (g.out_mode == 1)
+        {
+            r = r * g.scale[a]; // verts *= scale (MC:528)
+            r = r - 1.0f;       // verts -= 1 (MC:529)
+        }
+        else if (g.out_mode == 2)
+        {
+            r = fmaf(g.scale[a], r, g.corner[a]); // to_global_coordinates (visual_hull_cuda.cu:929-937)
+        }
+        out[a] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: cell case, completeness and edge sign changes per vertex
+// ---------------------------------------------------------------------------
+
+template <typename Lookup, typename V>
+__global__ __launch_bounds__(256) void
+mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* __restrict__ ntv)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= p.nverts)
+        return;
+    const VertexRef r = vertex_ref(p, i);
+    const McGrid& g = p.grids[r.scene];
+    const V iso = static_cast<V>(p.iso);
+    const V vg = value_at<V>(p, uint32_t(i));
+    const bool below_g = vg < iso;
+    uint32_t sc = 0u;
+    uint32_t ci = below_g ? 1u : 0u;
+    bool complete = (r.x + 1 < g.dims[0]) && (r.y + 1 < g.dims[1]) && (r.z + 1 < g.dims[2]); // MC:199
+    // neighbours along +x, +y, +z (case corners 1, 4, 3)
+    const uint32_t nb_corner[3] = { 1u, 4u, 3u };
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const uint32_t nx = r.x + (d == 0), ny = r.y + (d == 1), nz = r.z + (d == 2);
+        uint32_t pos;
+        bool found = false;
+        if (nx < g.dims[0] && ny < g.dims[1] && nz < g.dims[2])
+            found = lk.find(g, nx, ny, nz, pos);
+        if (found)
+        {
+            const bool below = value_at<V>(p, pos) < iso;
+            if (below != below_g)
+                sc |= 1u << d;
+            if (below)
+                ci |= 1u << nb_corner[d];
+        }
+        else
+            complete = false;
+    }
+    if (complete)
+    {
+        // remaining case corners 2 (1,0,1), 5 (1,1,0), 6 (1,1,1), 7 (0,1,1)
+        const uint32_t rest[4] = { 2u, 5u, 6u, 7u };
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const uint32_t o = c_corner_off[rest[k]];
+            uint32_t pos;
+            if (!lk.find(g, r.x + (o & 1u), r.y + ((o >> 1) & 1u), r.z + ((o >> 2) & 1u), pos))
+            {
+                complete = false;
+                break;
+            }
+            if (value_at<V>(p, pos) < iso)
+                ci |= 1u << rest[k];
+        }
+    }
+    if (!complete)
+        ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
+    info[i] = ci | (complete ? kInfoComplete : 0u) | (sc << kInfoSignShift);
+    ntv[i] = complete ? uint32_t(c_cases.count[ci]) : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: emitted edges per vertex
+// ---------------------------------------------------------------------------
+
+template <typename Lookup>
+__device__ __forceinline__ bool
+cell_complete(const McPlan& p, const Lookup& lk, const McGrid& g, const uint32_t* info, int32_t x, int32_t y,
+              int32_t z)
+{
+    if (x < 0 || y < 0 || z < 0)
+        return false;
+    uint32_t pos;
+    if (!lk.find(g, uint32_t(x), uint32_t(y), uint32_t(z), pos))
+        return false;
+    return (info[pos] & kInfoComplete) != 0u;
+}
+
+template <typename Lookup>
+__global__ __launch_bounds__(256) void
+mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t* __restrict__ emask)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= p.nverts)
+        return;
+    const uint32_t inf = info[i];
+    const uint32_t sc = (inf >> kInfoSignShift) & 7u;
+    uint32_t m = 0u;
+    if (sc != 0u)
+    {
+        const VertexRef r = vertex_ref(p, i);
+        const McGrid& g = p.grids[r.scene];
+        const int32_t x = int32_t(r.x), y = int32_t(r.y), z = int32_t(r.z);
+        const bool self = (inf & kInfoComplete) != 0u;
+        // cells sharing the edge from (x,y,z) along axis d: origins minus the two other unit vectors
+        if (sc & 1u)
+            if (self || cell_complete(p, lk, g, info, x, y - 1, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
+                cell_complete(p, lk, g, info, x, y - 1, z - 1))
+                m |= 1u;
+        if (sc & 2u)
+            if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
+                cell_complete(p, lk, g, info, x - 1, y, z - 1))
+                m |= 2u;
+        if (sc & 4u)
+            if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y - 1, z) ||
+                cell_complete(p, lk, g, info, x - 1, y - 1, z))
+                m |= 4u;
+    }
+    emask[i] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: output vertices (compact mesh)
+// ---------------------------------------------------------------------------
+
+template <typename Lookup, typename V>
+__global__ __launch_bounds__(256) void
+mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, const uint32_t* __restrict__ vbase,
+                   float* __restrict__ verts)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= p.nverts)
+        return;
+    const uint32_t m = emask[i];
+    if (m == 0u)
+        return;
+    const VertexRef r = vertex_ref(p, i);
+    const McGrid& g = p.grids[r.scene];
+    const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
+    const uint32_t p1[3] = { r.x, r.y, r.z };
+    uint32_t out = vbase[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        if (!(m & (1u << d)))
+            continue;
+        const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
+        uint32_t pos = 0;
+        lk.find(g, p2[0], p2[1], p2[2], pos); // present: the edge has a sign change
+        const float v2 = float(value_at<V>(p, pos));
+        float v[3];
+        interpolate_vertex(p.iso, p1, p2, v1, v2, v);
+        finish_vertex(g, v, verts + 3 * uint64_t(out));
+        ++out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 4: faces (and soup vertices)
+// ---------------------------------------------------------------------------
+
+template <typename Lookup, typename V>
+__global__ __launch_bounds__(256) void
+mc_faces_kernel(McPlan p,
+                Lookup lk,
+                const uint32_t* __restrict__ info,
+                const uint32_t* __restrict__ tbase,
+                const uint32_t* __restrict__ emask,
+                const uint32_t* __restrict__ vbase,
+                const uint32_t* __restrict__ scene_vstart,
+                const uint32_t* __restrict__ scene_tstart,
+                bool unique,
+                float* __restrict__ soup_verts,
+                int64_t* __restrict__ faces)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= p.nverts)
+        return;
+    const uint32_t inf = info[i];
+    if (!(inf & kInfoComplete))
+        return;
+    const uint32_t ci = inf & 0xffu;
+    const int count = c_cases.count[ci];
+    if (count == 0)
+        return;
+    const VertexRef r = vertex_ref(p, i);
+    const McGrid& g = p.grids[r.scene];
+    const uint32_t t0 = tbase[i];
+    const uint32_t ts = scene_tstart[r.scene];
+    const uint32_t vs = unique ? scene_vstart[r.scene] : 0u;
+    for (int k = 0; k < count; ++k)
+    {
+        const int e = c_cases.edges[ci][k];
+        const uint32_t lo = c_edge_low[e];
+        const int axis = c_edge_axis[e];
+        const uint32_t q1[3] = { r.x + (lo & 1u), r.y + ((lo >> 1) & 1u), r.z + ((lo >> 2) & 1u) };
+        uint32_t pos1 = 0;
+        lk.find(g, q1[0], q1[1], q1[2], pos1); // corner of a complete cell
+        const uint32_t tv = t0 + uint32_t(k) - ts; // triangle-vertex index within the scene
+        int64_t idx;
+        if (unique)
+        {
+            idx = int64_t(vbase[pos1] + __popc(emask[pos1] & ((1u << axis) - 1u)) - vs);
+        }
+        else
+        {
+            const uint32_t q2[3] = { q1[0] + (axis == 0), q1[1] + (axis == 1), q1[2] + (axis == 2) };
+            uint32_t pos2 = 0;
+            lk.find(g, q2[0], q2[1], q2[2], pos2);
+            float v[3];
+            interpolate_vertex(p.iso, q1, q2, float(value_at<V>(p, pos1)), float(value_at<V>(p, pos2)), v);
+            finish_vertex(g, v, soup_verts + 3 * uint64_t(t0 + uint32_t(k)));
+            idx = int64_t(tv); // faces = arange (MC:691-692)
+        }
+        int col = k % 3;
+        if (g.flip && col != 0)
+            col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
+        faces[3 * uint64_t((t0 + uint32_t(k)) / 3u) + col] = idx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+template <typename F>
+static void
+dispatch_mc(const McPlan& p, F&& f)
+{
+    BitmapLookup bl{ p.bitmap, p.prefix };
+    DenseLookup dl{};
+    if (p.dense)
+    {
+        if (p.value_type == kMcF64)
+            f(dl, double{});
+        else
+            f(dl, float{});
+    }
+    else
+    {
+        if (p.value_type == kMcF64)
+            f(bl, double{});
+        else
+            f(bl, float{});
+    }
+}
+
+void
+launch_mc_classify(const McPlan& p, uint32_t* info, uint32_t* ntv, hipStream_t st)
+{
+    if (p.nverts == 0)
+        return;
+    dispatch_mc(p, [&](auto lk, auto vt) {
+        using LK = decltype(lk);
+        using VT = decltype(vt);
+        mc_classify_kernel<LK, VT><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(p, lk, info, ntv);
+    });
+    THULL_LAUNCH_CHECK();
+}
+
+void
+launch_mc_edges(const McPlan& p, const uint32_t* info, uint32_t* emask, hipStream_t st)
+{
+    if (p.nverts == 0)
+        return;
+    dispatch_mc(p, [&](auto lk, auto) {
+        using LK = decltype(lk);
+        mc_edges_kernel<LK><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(p, lk, info, emask);
+    });
+    THULL_LAUNCH_CHECK();
+}
+
+void
+launch_mc_vertices(const McPlan& p, const uint32_t* emask, const uint32_t* vbase, float* verts, hipStream_t st)
+{
+    if (p.nverts == 0)
+        return;
+    dispatch_mc(p, [&](auto lk, auto vt) {
+        using LK = decltype(lk);
+        using VT = decltype(vt);
+        mc_vertices_kernel<LK, VT><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(p, lk, emask, vbase, verts);
+    });
+    THULL_LAUNCH_CHECK();
+}
+
+void
+launch_mc_faces(const McPlan& p,
+                const uint32_t* info,
+                const uint32_t* ntv_base,
+                const uint32_t* emask,
+                const uint32_t* vbase,
+                const uint32_t* scene_vstart,
+                const uint32_t* scene_tstart,
+                bool unique,
+                float* soup_verts,
+                int64_t* faces,
+                hipStream_t st)
+{
+    if (p.nverts == 0)
+        return;
+    dispatch_mc(p, [&](auto lk, auto vt) {
+        using LK = decltype(lk);
+        using VT = decltype(vt);
+        mc_faces_kernel<LK, VT><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(
+                p, lk, info, ntv_base, emask, vbase, scene_vstart, scene_tstart, unique, soup_verts, faces);
+    });
+    THULL_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Sparse COO input (MC:756-815): vertex bitmap of the nonzero entries
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void
+mark_coo_kernel(const int64_t* __restrict__ coo, uint64_t nnz, const McGrid* __restrict__ grid,
+                uint32_t* __restrict__ bitmap)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= nnz)
+        return;
+    const McGrid g = *grid;
+    const int64_t z = coo[1 * nnz + i], y = coo[2 * nnz + i], x = coo[3 * nnz + i];
+    if (x < 0 || y < 0 || z < 0 || x >= g.dims[0] || y >= g.dims[1] || z >= g.dims[2])
+        return; // rejected on the host; never write out of bounds
+    const uint64_t w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + uint64_t(y)) * g.row_words + uint64_t(x >> 5);
+    atomicOr(bitmap + w, 1u << uint32_t(x & 31));
+}
+
+void
+launch_mark_coo(const int64_t* coo, uint64_t nnz, const McGrid* grid, uint32_t* bitmap, hipStream_t st)
+{
+    if (nnz == 0)
+        return;
+    mark_coo_kernel<<<blocks_for(int64_t(nnz), 256), 256, 0, st>>>(coo, nnz, grid, bitmap);
+    THULL_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void
+extract_grid_vertices_kernel(const uint32_t* __restrict__ bitmap, const uint32_t* __restrict__ prefix,
+                             uint64_t nwords, const McGrid* __restrict__ grid, uint64_t* __restrict__ vertices)
+{
+    const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (w >= nwords)
+        return;
+    uint32_t word = bitmap[w];
+    if (word == 0u)
+        return;
+    const McGrid g = *grid;
+    const uint64_t row = w / g.row_words;
+    const uint32_t xw = uint32_t(w - row * g.row_words);
+    const uint32_t y = uint32_t(row % g.dims[1]), z = uint32_t(row / g.dims[1]);
+    uint32_t pos = prefix[w];
+    while (word != 0u)
+    {
+        const uint32_t b = __builtin_ctz(word);
+        vertices[pos++] = pack_key(0u, xw * 32u + b, y, z);
+        word &= word - 1u;
+    }
+}
+
+void
+launch_extract_grid_vertices(const uint32_t* bitmap, const uint32_t* prefix, uint64_t nwords, const McGrid* grid,
+                             uint64_t* vertices, hipStream_t st)
+{
+    if (nwords == 0)
+        return;
+    extract_grid_vertices_kernel<<<blocks_for(int64_t(nwords), 256), 256, 0, st>>>(bitmap, prefix, nwords, grid,
+                                                                                   vertices);
+    THULL_LAUNCH_CHECK();
+}
+
+} // namespace thull

@@ -1,0 +1,182 @@
+// Device-wide exclusive prefix sums over 32-bit counts, written for 64-lane
+// wavefronts: a tile of 256 threads x 16 items per block, wave prefix sums by
+// lane shuffles, one LDS slot per wave, and a three-kernel reduce / scan /
+// down-sweep that needs no inter-block waiting.
+//
+// Counts are read through a loader functor `L`, `uint32_t L::operator()(uint64_t i)`,
+// so callers scan derived quantities (popcounts, table lookups) without a
+// staging pass.
+#pragma once
+
+#include "common.h"
+
+namespace thull
+{
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Inclusive prefix sum across the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t
+wave_inclusive_sum(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d)
+            v += o;
+    }
+    return v;
+}
+
+// Exclusive prefix of v over the block (blockDim.x == kScanThreads); returns
+// the block total in `total`.  `lds` holds kScanThreads / 64 entries.
+__device__ __forceinline__ uint32_t
+block_exclusive_sum(uint32_t v, uint32_t* lds, uint32_t& total)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_sum(v);
+    if (lane == 63)
+        lds[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w)
+    {
+        uint32_t s = lds[w];
+        if (w < wave)
+            wave_off += s;
+        total += s;
+    }
+    __syncthreads();
+    return wave_off + inc - v;
+}
+
+template <typename L>
+__global__ __launch_bounds__(kScanThreads) void
+scan_reduce_kernel(L load, uint64_t n, uint32_t* block_sums)
+{
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint64_t i = base + k;
+        if (i < n)
+            s += load(i);
+    }
+    uint32_t total;
+    block_exclusive_sum(s, lds, total);
+    if (threadIdx.x == 0)
+        block_sums[blockIdx.x] = total;
+}
+
+// Single-block exclusive scan of the block sums; writes the grand total (64-bit).
+template <typename Tag = void>
+__global__ __launch_bounds__(1024) void
+scan_block_sums_kernel(uint32_t* sums, uint32_t nblocks, unsigned long long* total_out)
+{
+    __shared__ unsigned long long lds[1024 / 64];
+    const uint32_t per = (nblocks + 1023) / 1024;
+    const uint32_t begin = threadIdx.x * per;
+    unsigned long long local = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (begin + k < nblocks)
+            local += sums[begin + k];
+    // block scan of 64-bit partial sums
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long inc = local;
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        unsigned long long o = __shfl_up(inc, d, 64);
+        if (lane >= d)
+            inc += o;
+    }
+    if (lane == 63)
+        lds[wave] = inc;
+    __syncthreads();
+    unsigned long long off = 0, total = 0;
+    for (int w = 0; w < 1024 / 64; ++w)
+    {
+        if (w < wave)
+            off += lds[w];
+        total += lds[w];
+    }
+    unsigned long long run = off + inc - local;
+    for (uint32_t k = 0; k < per; ++k)
+        if (begin + k < nblocks)
+        {
+            uint32_t v = sums[begin + k];
+            sums[begin + k] = uint32_t(run);
+            run += v;
+        }
+    if (threadIdx.x == 0)
+        *total_out = total;
+}
+
+template <typename L>
+__global__ __launch_bounds__(kScanThreads) void
+scan_downsweep_kernel(L load, uint64_t n, const uint32_t* block_offsets, uint32_t* out)
+{
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint64_t i = base + k;
+        v[k] = (i < n) ? load(i) : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_offsets[blockIdx.x] + block_exclusive_sum(s, lds, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        uint64_t i = base + k;
+        if (i < n)
+            out[i] = run;
+        run += v[k];
+    }
+}
+
+// Exclusive scan of load(i), i in [0, n), into out[0, n).  `block_sums` must
+// hold scan_blocks(n) entries; the 64-bit total lands in *total_dev.
+inline uint32_t
+scan_blocks(uint64_t n)
+{
+    return uint32_t((n + kScanTile - 1) / kScanTile);
+}
+
+template <typename L>
+void
+exclusive_scan(L load, uint64_t n, uint32_t* out, uint32_t* block_sums, unsigned long long* total_dev, hipStream_t st)
+{
+    const uint32_t nb = scan_blocks(n);
+    if (nb == 0)
+    {
+        THULL_HIP(hipMemsetAsync(total_dev, 0, sizeof(unsigned long long), st));
+        return;
+    }
+    scan_reduce_kernel<<<nb, kScanThreads, 0, st>>>(load, n, block_sums);
+    THULL_LAUNCH_CHECK();
+    scan_block_sums_kernel<><<<1, 1024, 0, st>>>(block_sums, nb, total_dev);
+    THULL_LAUNCH_CHECK();
+    scan_downsweep_kernel<<<nb, kScanThreads, 0, st>>>(load, n, block_sums, out);
+    THULL_LAUNCH_CHECK();
+}
+
+struct LoadU32
+{
+    const uint32_t* p;
+    __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
+};
+
+} // namespace thull
