@@ -1,0 +1,222 @@
+"""Benchmark: visual hulls/s on one MI355X, batched vs per-scene, with roofline and CPU baseline.
+
+Workload (default, "mixed64"): 64 independent synthetic scenes (1..8 ellipsoids,
+4..48 cameras, 256/512/1024 px masks, level 6..9, masks_partial with p=0.3,
+blur k=7 sigma=1.5), masks resident in HBM.  One step = sparse blur of every
+scene's masks + visual hull of every scene.  `value` is the batched path (one
+carving launch per octree level across all scenes); the per-scene path (one
+pipeline per scene) is timed on the same data and every batched mesh is
+compared with its per-scene mesh.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import torchhull_amd as th  # noqa: E402
+from torchhull_amd import _lib, synthetic  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def scene_args(sc: synthetic.Scene, masks: torch.Tensor) -> dict:
+    return dict(masks=masks, transforms=sc.transforms, level=sc.level, cube_corner_bfl=sc.cube_corner_bfl,
+                cube_length=sc.cube_length, masks_partial=sc.masks_partial,
+                transforms_convention=sc.transforms_convention)
+
+
+def blur(sc: synthetic.Scene) -> torch.Tensor:
+    if sc.blur is None:
+        return sc.masks
+    k, s = sc.blur
+    return th.gaussian_blur(sc.masks, k, s, sparse=True, dtype=torch.float32)
+
+
+def step_batched(scenes):
+    return th.visual_hull_batch([scene_args(sc, blur(sc)) for sc in scenes], unique_verts=True)
+
+
+def step_per_scene(scenes):
+    return [th.visual_hull(**scene_args(sc, blur(sc)), unique_verts=True) for sc in scenes]
+
+
+def timed(fn, steps: int, warmup: int):
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def carve_profile(run_fn) -> dict:
+    """Runs run_fn once with event timing of the carving classification launches; per-level records."""
+    ctx = _lib.context(torch.cuda.current_device())
+    L = _lib.lib()
+    _lib.check(L.thull_profile_enable(ctx, 1))
+    try:
+        run_fn()
+        torch.cuda.synchronize()
+    finally:
+        _lib.check(L.thull_profile_enable(ctx, 0))
+    n = 16
+    ms = (ctypes.c_float * n)()
+    par, tests, nxt = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+    _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt))
+    levels = []
+    for i in range(n):
+        if par[i] == 0:
+            continue
+        # algorithmic bytes of carving step i: parents read (4 B), 4 integral reads of 4 B per camera
+        # test (early exit), children written (4 B)
+        b = 4 * par[i] + 16 * tests[i] + 4 * nxt[i]
+        levels.append(dict(step=i, level=i + 1, ms=float(ms[i]), parents=int(par[i]), camera_tests=int(tests[i]),
+                           children=int(nxt[i]), bytes=int(b)))
+    return dict(levels=levels)
+
+
+def roofline_from(levels: list[dict]) -> dict:
+    tot_b = sum(lv["bytes"] for lv in levels)
+    tot_ms = sum(lv["ms"] for lv in levels)
+    n = len(levels)
+    achieved = tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "carve_classify_kernel", "launches": n,
+            "bytes_per_launch": int(tot_b / max(n, 1)), "avg_launch_ms": round(tot_ms / max(n, 1), 4)}
+
+
+def cpu_baseline(scenes, gpu_meshes, budget_s: float) -> dict:
+    """The CPU oracle (C++ restatement of the reference path, OpenMP) on the first scenes until the budget."""
+    from oracle import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    done, t_total = 0, 0.0
+    mism = 0
+    for sc, (gv, gf) in zip(scenes, gpu_meshes):
+        m = sc.masks.cpu().numpy()
+        T = sc.transforms.cpu().numpy()
+        t0 = time.perf_counter()
+        if sc.blur is not None:
+            m = oracle.gaussian_blur_sparse(m.astype(np.float64), sc.blur[0], sc.blur[1], np.float32)
+        v, f = oracle.visual_hull(m, T, sc.level, sc.cube_corner_bfl, sc.cube_length, sc.masks_partial,
+                                  sc.transforms_convention, True)
+        t_total += time.perf_counter() - t0
+        done += 1
+        gv, gf = gv.cpu().numpy(), gf.cpu().numpy()
+        ok = gf.shape == f.shape and np.array_equal(gf, f) and gv.shape == v.shape and \
+            np.allclose(gv, v, rtol=0, atol=1e-6 * sc.cube_length, equal_nan=True)
+        mism += 0 if ok else 1
+        if t_total >= budget_s:
+            break
+    return {"value": round(done / t_total, 4), "unit": "hulls/s", "cores": cores, "kind": "port",
+            "sample": f"first {done} scenes of the mixed64 workload (blur + carve + field + MC), "
+                      f"{t_total:.1f} s on {cores} threads ({platform.processor() or platform.machine()})",
+            "oracle_mesh_mismatches": mism, "oracle_scenes_checked": done}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the level-10 carving measurement of config 2")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(scenes are independent, so N GPUs would each run their own batch)", flush=True)
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    log("rendering mixed workload scenes on the GPU ...")
+    scenes = synthetic.mixed_scenes(args.scenes, device=dev)
+    torch.cuda.synchronize()
+    mask_bytes = sum(sc.masks.numel() * 4 for sc in scenes)
+    log(f"  {len(scenes)} scenes, {mask_bytes / 2**30:.2f} GiB of masks")
+
+    t_b, meshes_b = timed(lambda: step_batched(scenes), args.steps, args.warmup)
+    log(f"batched: {1e3 * t_b / args.steps:.2f} ms/step")
+    t_s, meshes_s = timed(lambda: step_per_scene(scenes), args.steps, args.warmup)
+    log(f"per-scene: {1e3 * t_s / args.steps:.2f} ms/step")
+
+    mismatch = 0
+    for (vb, fb), (vs, fs) in zip(meshes_b, meshes_s):
+        if not (torch.equal(fb, fs) and torch.equal(vb.view(torch.int32), vs.view(torch.int32))):
+            mismatch += 1
+
+    prof = carve_profile(lambda: step_batched(scenes))
+    roof = roofline_from(prof["levels"])
+
+    result = {
+        "metric": "visual hulls/s batched vs per-scene on 1 GPU (exact mesh match) + p50/p99 latency",
+        "value": round(len(scenes) * args.steps / t_b, 3),
+        "unit": "hulls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_b / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid unions, exact ray-ellipsoid masks rendered on the GPU)",
+        "config": {"workload": "mixed64: 64 scenes, 1-8 ellipsoids, 4-48 cameras, 256/512/1024 px, level 6-9, "
+                               "masks_partial p=0.3, blur k7 sigma1.5, opengl", "scenes": len(scenes),
+                   "mask_gib": round(mask_bytes / 2**30, 3)},
+        "per_scene_hulls_per_s": round(len(scenes) * args.steps / t_s, 3),
+        "batched_over_per_scene": round(t_s / t_b, 3),
+        "exact_match": {"batched_vs_per_scene_mismatches": mismatch, "scenes": len(scenes)},
+        "roofline": roof,
+        "carve_levels": prof["levels"],
+    }
+
+    if not args.no_c2:
+        log("config 2 (32 x 2048^2, level 10, opencv, partial): rendering ...")
+        sc2 = synthetic.config_scene(2, 0, device=dev)
+        m2 = blur(sc2)
+        run2 = lambda: th.visual_hull(**scene_args(sc2, m2), unique_verts=True)  # noqa: E731
+        t2, _ = timed(run2, 3, 1)
+        p2 = carve_profile(run2)
+        last = p2["levels"][-1]
+        result["config2_level10_carve"] = {
+            "level": last["level"], "ms": round(last["ms"], 4), "bytes": last["bytes"],
+            "camera_tests": last["camera_tests"], "achieved_GBs": round(last["bytes"] / (last["ms"] * 1e-3) / 1e9, 1),
+            "frac": round(last["bytes"] / (last["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "camera_tests_per_s": round(last["camera_tests"] / (last["ms"] * 1e-3), 1),
+            "hull_ms": round(1e3 * t2 / 3, 3)}
+        del m2, sc2
+
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline(scenes, meshes_b, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,6 +113,14 @@ int thull_visual_hull_fetch_candidates(thull_ctx* ctx, int32_t level, int64_t* o
  * `max_levels` entries.  Instrumentation for throughput accounting. */
 int thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_tests, int64_t* children);
 
+/* Optional HIP-event timing of every carving classification launch of later runs
+ * (no counterpart in the reference; instrumentation for roofline accounting). */
+int thull_profile_enable(thull_ctx* ctx, int32_t enabled);
+/* Per carving step i of the last run: classification kernel time (ms, 0 when
+ * profiling is off), parent cells, camera tests and resulting child cells. */
+int thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests,
+                        int64_t* next);
+
 /* Marching cubes on a dense volume [1,D,H,W] (value dtype THULL_F32 or THULL_F64)
  * or a coalesced sparse COO volume (indices int64 [4,nnz], values [nnz]).
  * return_local_coords maps to [-1,1]; otherwise grid units.  Replaces

@@ -153,6 +153,24 @@ struct Context
     float* verts = nullptr;
     int64_t* faces = nullptr;
 
+    // optional HIP-event timing of the carving classification launches
+    bool profile = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<float> carve_ms;
+    std::vector<int64_t> carve_parents, carve_next;
+
+    hipEvent_t
+    event(size_t i)
+    {
+        while (ev_pool.size() <= i)
+        {
+            hipEvent_t e;
+            THULL_HIP(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        return ev_pool[i];
+    }
+
     // last marching-cubes run
     float* mc_verts = nullptr;
     int64_t* mc_faces = nullptr;
@@ -296,6 +314,9 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     ctx.counts.assign(n, thull_vh_counts{});
     ctx.level_tests.clear();
     ctx.level_children.clear();
+    ctx.carve_ms.clear();
+    ctx.carve_parents.clear();
+    ctx.carve_next.clear();
     ctx.vertices = nullptr;
     ctx.values = nullptr;
     ctx.verts = nullptr;
@@ -377,7 +398,11 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         uint32_t* boff = pool.alloc<uint32_t>(nb);
         uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nb) + 1);
         THULL_HIP(hipMemsetAsync(dev_tests, 0, sizeof(unsigned long long), st));
+        if (ctx.profile)
+            THULL_HIP(hipEventRecord(ctx.event(2 * i), st));
         launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests, st);
+        if (ctx.profile)
+            THULL_HIP(hipEventRecord(ctx.event(2 * i + 1), st));
         exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st);
         THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         THULL_HIP(hipMemcpyAsync(ctx.stage->p + 1, dev_tests, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -385,6 +410,14 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         const uint64_t next = ctx.stage->p[0];
         ctx.level_tests.push_back(int64_t(ctx.stage->p[1]));
         ctx.level_children.push_back(int64_t(count) * 8);
+        ctx.carve_parents.push_back(int64_t(count));
+        ctx.carve_next.push_back(int64_t(next));
+        if (ctx.profile)
+        {
+            float ms = 0.0f;
+            THULL_HIP(hipEventElapsedTime(&ms, ctx.event(2 * i), ctx.event(2 * i + 1)));
+            ctx.carve_ms.push_back(ms);
+        }
         check_u32(next, "octree level size");
         uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(next, 1));
         launch_carve_scatter(list, count, mask, boff, children, st);
@@ -719,6 +752,36 @@ thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_test
                 camera_tests[i] = have ? c.level_tests[i] : 0;
             if (children)
                 children[i] = have ? c.level_children[i] : 0;
+        }
+    });
+}
+
+int
+thull_profile_enable(thull_ctx* ctx, int32_t enabled)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        ctx->c.profile = enabled != 0;
+    });
+}
+
+int
+thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests, int64_t* next)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        for (int32_t i = 0; i < max_levels; ++i)
+        {
+            const bool have = i < int32_t(c.carve_parents.size());
+            if (ms)
+                ms[i] = (have && i < int32_t(c.carve_ms.size())) ? c.carve_ms[i] : 0.0f;
+            if (parents)
+                parents[i] = have ? c.carve_parents[i] : 0;
+            if (tests)
+                tests[i] = have ? c.level_tests[i] : 0;
+            if (next)
+                next[i] = have ? c.carve_next[i] : 0;
         }
     });
 }
