@@ -52,7 +52,10 @@ def blur(sc: synthetic.Scene) -> torch.Tensor:
 
 
 def step_batched(scenes):
-    return th.visual_hull_batch([scene_args(sc, blur(sc)) for sc in scenes], unique_verts=True)
+    # one blur launch per pass for all scenes (they share k and sigma), then one pipeline for all hulls
+    blurred = th.gaussian_blur_batch([sc.masks for sc in scenes], scenes[0].blur[0], scenes[0].blur[1],
+                                     sparse=True, dtype=torch.float32)
+    return th.visual_hull_batch([scene_args(sc, m) for sc, m in zip(scenes, blurred)], unique_verts=True)
 
 
 def step_per_scene(scenes):

@@ -169,6 +169,21 @@ int thull_gaussian_blur(thull_ctx* ctx,
                         int32_t out_dtype,
                         void* stream);
 
+/* The same blur for `n` image tensors of one input dtype in one launch per pass:
+ * images[i] / outs[i] are device [B,H,W] tensors with sizes[3i..3i+2] = (B, H, W).
+ * Batched form of gaussian_blur_cuda (src/torchhull/_C/src/gaussian_blur_cuda.cu:509-518). */
+int thull_gaussian_blur_batch(thull_ctx* ctx,
+                              int32_t n,
+                              const void* const* images,
+                              int32_t in_dtype,
+                              const int64_t* sizes,
+                              int32_t kernel_size,
+                              float sigma,
+                              int32_t sparse,
+                              void* const* outs,
+                              int32_t out_dtype,
+                              void* stream);
+
 /* float32 integral image of masks [B,H,W] (dtype THULL_F32/U8/BOOL); equal to the
  * reference's integral_image (src/torchhull/_C/src/visual_hull_cuda.cu:76-90). */
 int thull_integral_image(thull_ctx* ctx,

@@ -69,6 +69,9 @@ SCENES = {
     "c2_gl_full": (2, {"n_cameras": 10, "resolution": 256, "level": 6, "convention": "opengl",
                        "masks_partial": False}),
     "c0_partial": (0, {"level": 6, "masks_partial": True}),
+    # large enough that the last levels use the one-thread-per-parent carving kernel
+    "c1_l8": (1, {"n_cameras": 16, "resolution": 512, "level": 8}),
+    "c2_l8": (2, {"n_cameras": 12, "resolution": 512, "level": 8}),
 }
 
 
@@ -109,9 +112,9 @@ def test_visual_hull_matches_oracle(name, unique_verts):
                                 sc.transforms_convention, unique_verts, field=field)
     assert_mesh_equal(v, f, ov, of, atol=1e-6 * sc.cube_length)
     if unique_verts:
-        # with masks_partial, regions few cameras see stay "unknown" and the hull may reach the
-        # cube faces, where no complete cell closes it; full-mask hulls are closed
-        if not sc.masks_partial:
+        # with masks_partial, or cameras inside the cube (config 2), parts of the cube are unseen and
+        # the hull may reach the cube faces, where no complete cell closes it
+        if not sc.masks_partial and not name.startswith("c2"):
             assert is_closed_manifold(f.cpu().numpy())
     else:
         assert v.shape[0] == 3 * f.shape[0]
@@ -158,6 +161,15 @@ def test_visual_hull_empty_and_full_masks(fill, partial):
     for level in range(0, 7):
         v, f = th.visual_hull(m, sc.transforms.to(DEV), level, (-1.1, -1.1, -1.1), 2.2, partial, "opengl", True)
         assert v.shape[0] == 0 and f.shape[0] == 0
+
+
+def test_blur_batch_equals_single():
+    th = _th()
+    ims = [_scene(n).masks.to(DEV) for n in ("c0_l5", "c2_small", "c1_small")]
+    outs = th.gaussian_blur_batch(ims, 9, 2.0, sparse=True, dtype=torch.float32)
+    for im, o in zip(ims, outs):
+        ref = th.gaussian_blur(im, 9, 2.0, sparse=True, dtype=torch.float32)
+        assert torch.equal(o.view(torch.int32), ref.view(torch.int32))
 
 
 def test_visual_hull_batch_equals_single():

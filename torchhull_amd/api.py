@@ -326,6 +326,41 @@ def gaussian_blur(images: torch.Tensor, kernel_size: int, sigma: float, sparse: 
         return out
 
 
+def gaussian_blur_batch(images: Sequence[torch.Tensor], kernel_size: int, sigma: float, sparse: bool = True,
+                        dtype: torch.dtype | None = None) -> list[torch.Tensor]:
+    """gaussian_blur of many [B,H,W,1] tensors (one input dtype, one device) in one launch per pass.
+
+    Each result equals gaussian_blur(images[i], kernel_size, sigma, sparse, dtype)."""
+    _check(len(images) > 0, "len(images) > 0")
+    device = images[0].device
+    _require_gpu(images[0])
+    _check(int(kernel_size) > 0, "kernel_size > 0")
+    _check(int(kernel_size) % 2 == 1, "kernel_size % 2 == 1")
+    _check(float(sigma) > 0.0, "sigma > 0.f")
+    in_dtype = images[0].dtype
+    out_dtype = in_dtype if dtype is None else dtype
+    _check(out_dtype in (torch.float16, torch.float32, torch.float64), "isFloatingType(dtype)")
+    if in_dtype not in _DTYPE_CODES:
+        raise RuntimeError(f"gaussian_blur: unsupported input dtype {in_dtype}")
+    srcs, outs, sizes = [], [], []
+    for im in images:
+        _check(im.device == device and im.dtype == in_dtype, "one device and dtype per batch")
+        _check(im.dim() == 4 and im.size(3) == 1, "images [B,H,W,1]")
+        srcs.append(im[..., 0].contiguous())
+        outs.append(torch.empty(tuple(im.shape), dtype=out_dtype, device=device))
+        sizes.extend(int(v) for v in im.shape[:3])
+    n = len(images)
+    src_p = (ctypes.c_void_p * n)(*[t.data_ptr() for t in srcs])
+    out_p = (ctypes.c_void_p * n)(*[t.data_ptr() for t in outs])
+    sz = (ctypes.c_int64 * (3 * n))(*sizes)
+    ctx = _lib.context(device.index if device.index is not None else torch.cuda.current_device())
+    with torch.cuda.device(device):
+        _lib.check(_lib.lib().thull_gaussian_blur_batch(ctx, n, src_p, _DTYPE_CODES[in_dtype], sz, int(kernel_size),
+                                                        float(sigma), int(bool(sparse)), out_p,
+                                                        _DTYPE_CODES[out_dtype], _stream(device)))
+    return outs
+
+
 def store_curve_network(filename: str, curve_network: tuple[torch.Tensor, torch.Tensor],
                         verbose: bool = False) -> None:
     """Writes a curve network (vertices [V,3], edges [E,2]) as OBJ or binary little-endian PLY (io.cpp:11-127)."""

@@ -15,6 +15,8 @@
 // float and double, per-operation binary16 rounding for half) and its final
 // division by the weight sum; every other pixel is the converted input.
 
+#include <vector>
+
 #include <hip/hip_fp16.h>
 
 #include "kernels.h"
@@ -106,106 +108,190 @@ reflect_index(int64_t i, int64_t n)
     return (flips % 2 == 0) ? rem : p - rem;
 }
 
-template <typename Ti>
-__global__ __launch_bounds__(256) void
-blur_tile_flags_kernel(const Ti* __restrict__ img, int64_t H, int64_t W, int32_t ts, int64_t TH, int64_t TW,
-                       uint8_t* __restrict__ flags)
+// One blur job: B images of H x W.  Blocks of all jobs are flattened into one
+// launch per pass; `*_begin` is the job's first block.
+struct BlurJob
 {
-    const int64_t tx = int64_t(blockIdx.x) * 16 + (threadIdx.x & 15);
-    const int64_t ty = int64_t(blockIdx.y) * 16 + (threadIdx.x >> 4);
-    const int64_t b = blockIdx.z;
-    if (tx >= TW || ty >= TH)
-        return;
-    const Ti* I = img + b * H * W;
-    uint8_t edge = 0;
-    for (int32_t ly = 0; ly < ts && !edge; ++ly)
+    const void* in;
+    void* out;
+    uint8_t* flags;      // B * TH * TW edge-tile flags, then B * TH * TW band-tile flags (sparse)
+    int32_t B, H, W;
+    int32_t TH, TW;      // tile grid (W / ts + 1, H / ts + 1)
+    uint32_t bx, by;     // 64 x RB pixel blocks per image (x, y)
+    uint32_t tb;         // tile blocks per image (256 tiles each) of the dilation pass
+    uint32_t pad_;
+    uint64_t p_begin;    // first pixel block of this job
+    uint64_t t_begin;    // first tile block of this job
+};
+
+__device__ __forceinline__ int32_t
+find_blur_job(const BlurJob* jobs, int32_t n, uint64_t blk, bool tiles)
+{
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi)
     {
-        const int64_t y = ty * ts + ly;
-        if (y >= H)
-            break;
-        for (int32_t lx = 0; lx < ts; ++lx)
-        {
-            const int64_t x = tx * ts + lx;
-            if (x >= W)
-                break;
-            const Ti c = I[y * W + x];
-            const Ti r = (x + 1 < W) ? I[y * W + x + 1] : c;
-            const Ti d = (y + 1 < H) ? I[(y + 1) * W + x] : c;
-            if (differs(r, c) || differs(d, c))
-            {
-                edge = 1;
-                break;
-            }
-        }
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if ((tiles ? jobs[mid].t_begin : jobs[mid].p_begin) <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    flags[(b * TH + ty) * TW + tx] = edge;
+    return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <typename Ti, typename To>
+constexpr int kBlurBX = 64; // pixel block: 64 columns x RB rows, 256 threads
+
+// Edge pixels (GB:116-142) marked into their tile's flag.  Every writer
+// stores the same value 1, so the plain byte stores need no atomics; flags
+// are zeroed beforehand.
+template <typename Ti, int RB>
 __global__ __launch_bounds__(256) void
-blur_output_kernel(const Ti* __restrict__ img,
-                   int64_t H,
-                   int64_t W,
-                   int32_t k,
-                   int32_t ts,
-                   int64_t TH,
-                   int64_t TW,
-                   const uint8_t* __restrict__ flags,
-                   bool dense,
-                   bool divide,
-                   const To* __restrict__ weights,
-                   To* __restrict__ out)
+blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t ts)
 {
-    const int64_t x = int64_t(blockIdx.x) * 64 + (threadIdx.x & 63);
-    const int64_t y = int64_t(blockIdx.y) * 4 + (threadIdx.x >> 6);
-    const int64_t b = blockIdx.z;
-    if (x >= W || y >= H)
+    const uint64_t blk = blockIdx.x;
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const uint32_t local = uint32_t(blk - j.p_begin);
+    const uint32_t per = j.bx * j.by;
+    const int32_t b = int32_t(local / per);
+    const uint32_t r = local - uint32_t(b) * per;
+    const int32_t by = int32_t(r / j.bx), bx = int32_t(r - uint32_t(by) * j.bx);
+    const int32_t H = j.H, W = j.W;
+    const int32_t x = bx * kBlurBX + int32_t(threadIdx.x & 63);
+    if (x >= W)
         return;
-    const Ti* I = img + b * H * W;
-    bool band = dense;
-    if (!band && flags != nullptr)
+    const Ti* I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
+    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW;
+    const int32_t tx = x / ts;
+#pragma unroll 4
+    for (int32_t q = 0; q < RB / 4; ++q)
     {
-        const int64_t tx = x / ts, ty = y / ts;
-        for (int64_t dy = -1; dy <= 1 && !band; ++dy)
-            for (int64_t dx = -1; dx <= 1; ++dx)
-            {
-                const int64_t ux = tx + dx, uy = ty + dy;
-                if (ux < 0 || uy < 0 || ux >= TW || uy >= TH)
-                    continue;
-                if (flags[(b * TH + uy) * TW + ux])
-                {
-                    band = true;
-                    break;
-                }
-            }
+        const int32_t y = by * RB + int32_t(threadIdx.x >> 6) + 4 * q;
+        if (y >= H)
+            break;
+        const int64_t o = int64_t(y) * W + x;
+        const Ti c = I[o];
+        const Ti rr = (x + 1 < W) ? I[o + 1] : c;
+        const Ti d = (y + 1 < H) ? I[o + W] : c;
+        if (differs(rr, c) || differs(d, c))
+            F[(y / ts) * j.TW + tx] = 1;
     }
-    To result;
-    if (!band)
+}
+
+// Band tiles: the 3x3 dilation of the edge tiles, clamped to the tile grid
+// (GB:154-176).  A pixel is a band pixel iff its tile is a band tile.
+__global__ __launch_bounds__(256) void
+blur_band_tiles_kernel(const BlurJob* __restrict__ jobs, int32_t njobs)
+{
+    const uint64_t blk = blockIdx.x;
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, true)];
+    const uint64_t t = uint64_t(blk - j.t_begin) * 256 + threadIdx.x;
+    const uint64_t plane = uint64_t(j.TH) * j.TW;
+    if (t >= plane * j.B)
+        return;
+    const int32_t b = int32_t(t / plane);
+    const int32_t rem = int32_t(t - uint64_t(b) * plane);
+    const int32_t ty = rem / j.TW, tx = rem - ty * j.TW;
+    const uint8_t* E = j.flags + uint64_t(b) * plane;
+    uint8_t band = 0;
+    for (int32_t uy = max(ty - 1, 0); uy <= min(ty + 1, j.TH - 1); ++uy)
+        for (int32_t ux = max(tx - 1, 0); ux <= min(tx + 1, j.TW - 1); ++ux)
+            band |= E[uy * j.TW + ux];
+    j.flags[plane * j.B + t] = band;
+}
+
+// Band pixels of a 64 x RB block are convolved from an LDS tile with a halo
+// of k/2 (reflect-padded at the image border); blocks without band pixels
+// only convert and copy.  `norm` is the reference's sum of the weights in tap
+// order and output precision, the same for every pixel.
+template <typename Ti, typename To, int RB>
+__global__ __launch_bounds__(256) void
+blur_tile_kernel(const BlurJob* __restrict__ jobs,
+                 int32_t njobs,
+                 int32_t k,
+                 int32_t ts,
+                 bool sparse,
+                 const To* __restrict__ weights,
+                 To norm)
+{
+    extern __shared__ unsigned char smem_raw[];
+    To* tile = reinterpret_cast<To*>(smem_raw);
+    constexpr int P = RB / 4; // pixels per thread
+    const uint64_t blk = blockIdx.x;
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const uint32_t local = uint32_t(blk - j.p_begin);
+    const uint32_t per = j.bx * j.by;
+    const int32_t b = int32_t(local / per);
+    const uint32_t rr = local - uint32_t(b) * per;
+    const int32_t by = int32_t(rr / j.bx), bx = int32_t(rr - uint32_t(by) * j.bx);
+    const int32_t H = j.H, W = j.W;
+    const int32_t h = k / 2;
+    const int32_t LW = kBlurBX + 2 * h, LH = RB + 2 * h;
+    const int32_t x0 = bx * kBlurBX, y0 = by * RB;
+    const Ti* I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
+    To* O = static_cast<To*>(j.out) + int64_t(b) * H * W;
+    const int32_t lx = int32_t(threadIdx.x & 63), ly0 = int32_t(threadIdx.x >> 6);
+    const int32_t x = x0 + lx;
+    const bool dense = !sparse && k > 1;
+    const bool use_flags = sparse && k > 1;
+    const uint8_t* band_tiles = use_flags ? j.flags + uint64_t(j.B + b) * j.TH * j.TW : nullptr;
+    uint32_t band = 0u; // bit q: pixel (x, y0 + ly0 + 4q)
+    if (x < W)
     {
-        result = convert_in<To>(I[y * W + x]); // images.clone().to(dtype) (GB:444)
-    }
-    else
-    {
-        const int32_t h = k / 2;
-        const bool interior = (x - h >= 0) && (x + h < W) && (y - h >= 0) && (y + h < H);
-        To acc = OutOps<To>::zero();
-        To norm = OutOps<To>::zero();
-        for (int32_t i = -h; i <= h; ++i)
+        for (int32_t q = 0; q < P; ++q)
         {
-            const int64_t xi = interior ? x + i : reflect_index(x + i, W);
-            for (int32_t j = -h; j <= h; ++j)
-            {
-                const int64_t yj = interior ? y + j : reflect_index(y + j, H);
-                const To v = convert_in<To>(I[yj * W + xi]);
-                const To wt = weights[(i + h) * k + (j + h)];
-                acc = OutOps<To>::mac(acc, wt, v);
-                if (divide)
-                    norm = OutOps<To>::add(norm, wt);
-            }
+            const int32_t y = y0 + ly0 + 4 * q;
+            if (y >= H)
+                break;
+            const bool bq = dense || (use_flags && band_tiles[(y / ts) * j.TW + x / ts] != 0);
+            band |= bq ? (1u << q) : 0u;
         }
-        result = divide ? OutOps<To>::div(acc, norm) : acc;
     }
-    out[(b * H + y) * W + x] = result;
+    if (!__syncthreads_or(band != 0u))
+    {
+        if (x < W)
+            for (int32_t q = 0; q < P; ++q)
+            {
+                const int32_t y = y0 + ly0 + 4 * q;
+                if (y >= H)
+                    break;
+                O[int64_t(y) * W + x] = convert_in<To>(I[int64_t(y) * W + x]); // images.clone().to(dtype) (GB:444)
+            }
+        return;
+    }
+    // stage the block and its halo, converted to the output type (GB:221)
+    for (int32_t q = threadIdx.x; q < LW * LH; q += 256)
+    {
+        const int32_t ty = q / LW, tx = q - ty * LW;
+        const int64_t yy = reflect_index(y0 + ty - h, H), xx = reflect_index(x0 + tx - h, W);
+        tile[q] = convert_in<To>(I[yy * W + xx]);
+    }
+    __syncthreads();
+    if (x >= W)
+        return;
+    for (int32_t q = 0; q < P; ++q)
+    {
+        const int32_t ly = ly0 + 4 * q;
+        const int32_t y = y0 + ly;
+        if (y >= H)
+            break;
+        To result;
+        if (!(band & (1u << q)))
+            result = tile[(ly + h) * LW + lx + h];
+        else
+        {
+            // GB:217-230: x offset i outer, y offset j inner
+            To acc = OutOps<To>::zero();
+            for (int32_t i = -h; i <= h; ++i)
+            {
+                const To* col = tile + (lx + h + i);
+                const To* wcol = weights + (i + h) * k;
+                for (int32_t jj = -h; jj <= h; ++jj)
+                    acc = OutOps<To>::mac(acc, wcol[jj + h], col[(ly + h + jj) * LW]);
+            }
+            result = sparse ? OutOps<To>::div(acc, norm) : acc;
+        }
+        O[int64_t(y) * W + x] = result;
+    }
 }
 
 int64_t
@@ -214,89 +300,145 @@ blur_tile_flag_bytes(int64_t B, int64_t H, int64_t W, int kernel_size)
     const int64_t ts = kernel_size / 2;
     if (ts <= 0)
         return 0;
-    return B * (H / ts + 1) * (W / ts + 1);
+    return 2 * B * (H / ts + 1) * (W / ts + 1); // edge tiles, then band tiles
+}
+
+size_t
+blur_job_bytes(size_t n)
+{
+    return n * sizeof(BlurJob);
+}
+
+template <typename Ti, typename To, int RB>
+static void
+run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, hipStream_t st)
+{
+    const int32_t ts = k / 2;
+    const bool use_band = sparse && k > 1;
+    uint64_t pb = 0, tbk = 0;
+    for (auto& j : jobs)
+    {
+        j.bx = blocks_for(j.W, kBlurBX);
+        j.by = blocks_for(j.H, RB);
+        j.p_begin = pb;
+        pb += uint64_t(j.bx) * j.by * j.B;
+        if (use_band)
+        {
+            j.TH = j.H / ts + 1;
+            j.TW = j.W / ts + 1;
+            j.tb = uint32_t(blocks_for(int64_t(j.TH) * j.TW * j.B, 256));
+            j.t_begin = tbk;
+            tbk += j.tb;
+            THULL_HIP(hipMemsetAsync(j.flags, 0, size_t(j.B) * j.TH * j.TW, st));
+        }
+    }
+    if (pb == 0)
+        return;
+    THULL_CHECK(pb < (uint64_t(1) << 31) && tbk < (uint64_t(1) << 31), "blur batch too large");
+    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
+    const int32_t n = int32_t(jobs.size());
+    if (use_band)
+    {
+        blur_edge_flags_kernel<Ti, RB><<<unsigned(pb), 256, 0, st>>>(djobs, n, ts);
+        THULL_LAUNCH_CHECK();
+        blur_band_tiles_kernel<<<unsigned(tbk), 256, 0, st>>>(djobs, n);
+        THULL_LAUNCH_CHECK();
+    }
+    const int32_t h = k / 2;
+    const size_t lds = size_t(kBlurBX + 2 * h) * (RB + 2 * h) * sizeof(To);
+    blur_tile_kernel<Ti, To, RB><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, ts > 0 ? ts : 1, sparse, weights, norm);
+    THULL_LAUNCH_CHECK();
+    // the job table is pageable host memory read by an async copy
+    THULL_HIP(hipStreamSynchronize(st));
 }
 
 template <typename Ti, typename To>
 static void
-run_blur(const Ti* img, int64_t B, int64_t H, int64_t W, int k, bool sparse, const To* weights, uint8_t* flags,
-         To* out, hipStream_t st)
+run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, hipStream_t st)
 {
-    const int32_t ts = k / 2;
-    int64_t TH = 0, TW = 0;
-    const bool use_band = sparse && k > 1;
-    if (use_band)
-    {
-        TH = H / ts + 1;
-        TW = W / ts + 1;
-        dim3 gf(blocks_for(TW, 16), blocks_for(TH, 16), unsigned(B));
-        blur_tile_flags_kernel<Ti><<<gf, 256, 0, st>>>(img, H, W, ts, TH, TW, flags);
-        THULL_LAUNCH_CHECK();
-    }
-    dim3 go(blocks_for(W, 64), blocks_for(H, 4), unsigned(B));
-    const bool dense = !sparse && k > 1;
-    blur_output_kernel<Ti, To><<<go, 256, 0, st>>>(img, H, W, k, ts > 0 ? ts : 1, TH, TW, use_band ? flags : nullptr,
-                                                   dense, sparse, weights, out);
-    THULL_LAUNCH_CHECK();
+    const int32_t h = k / 2;
+    if (size_t(kBlurBX + 2 * h) * (64 + 2 * h) * sizeof(To) <= 48 * 1024)
+        run_blur_rb<Ti, To, 64>(jobs, djobs, k, sparse, weights, norm, st);
+    else if (size_t(kBlurBX + 2 * h) * (16 + 2 * h) * sizeof(To) <= 48 * 1024)
+        run_blur_rb<Ti, To, 16>(jobs, djobs, k, sparse, weights, norm, st);
+    else
+        throw Error("kernel_size too large for the LDS tile");
 }
 
 template <typename Ti>
 static void
-run_blur_out(const Ti* img, int64_t B, int64_t H, int64_t W, int k, bool sparse, const void* weights,
-             uint8_t* flags, void* out, int32_t out_dtype, hipStream_t st)
+run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const void* weights, double norm,
+             int32_t out_dtype, hipStream_t st)
 {
     switch (out_dtype)
     {
     case kF16:
-        run_blur<Ti, __half>(img, B, H, W, k, sparse, static_cast<const __half*>(weights), flags,
-                             static_cast<__half*>(out), st);
+        run_blur<Ti, __half>(jobs, djobs, k, sparse, static_cast<const __half*>(weights), __float2half(float(norm)), st);
         break;
     case kF32:
-        run_blur<Ti, float>(img, B, H, W, k, sparse, static_cast<const float*>(weights), flags,
-                            static_cast<float*>(out), st);
+        run_blur<Ti, float>(jobs, djobs, k, sparse, static_cast<const float*>(weights), float(norm), st);
         break;
     case kF64:
-        run_blur<Ti, double>(img, B, H, W, k, sparse, static_cast<const double*>(weights), flags,
-                             static_cast<double*>(out), st);
+        run_blur<Ti, double>(jobs, djobs, k, sparse, static_cast<const double*>(weights), norm, st);
         break;
     default:
         throw Error("gaussian_blur: output dtype must be floating point");
     }
 }
 
-// `weights` is a device table of k*k weights in the output type.
+// Blurs a batch of images of one input dtype into outputs of one output dtype.
+// `weights` is a device table of k*k weights in the output type; `norm` the
+// weight sum in the output type (exactly representable in double).
 void
-launch_gaussian_blur_impl(const void* images, int32_t in_dtype, int64_t B, int64_t H, int64_t W, int k, bool sparse,
-                          const void* weights, void* out, int32_t out_dtype, uint8_t* flags, hipStream_t st)
+launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int k, bool sparse,
+                           const void* weights, double norm, int32_t out_dtype, void* job_buf, hipStream_t st)
 {
-    if (B == 0 || H == 0 || W == 0)
+    std::vector<BlurJob> jobs;
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const BlurImage& im = imgs[i];
+        if (im.B == 0 || im.H == 0 || im.W == 0)
+            continue;
+        THULL_CHECK(im.H < (int64_t(1) << 31) && im.W < (int64_t(1) << 31) && im.H * im.W < (int64_t(1) << 40),
+                    "image too large");
+        BlurJob j{};
+        j.in = im.in;
+        j.out = im.out;
+        j.flags = im.flags;
+        j.B = int32_t(im.B);
+        j.H = int32_t(im.H);
+        j.W = int32_t(im.W);
+        jobs.push_back(j);
+    }
+    if (jobs.empty())
         return;
+    BlurJob* djobs = static_cast<BlurJob*>(job_buf);
     switch (in_dtype)
     {
     case kBool:
     case kU8:
-        run_blur_out(static_cast<const uint8_t*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<uint8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kI8:
-        run_blur_out(static_cast<const int8_t*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<int8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kI16:
-        run_blur_out(static_cast<const int16_t*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<int16_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kI32:
-        run_blur_out(static_cast<const int32_t*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<int32_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kI64:
-        run_blur_out(static_cast<const int64_t*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<int64_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kF16:
-        run_blur_out(static_cast<const __half*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<__half>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kF32:
-        run_blur_out(static_cast<const float*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<float>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     case kF64:
-        run_blur_out(static_cast<const double*>(images), B, H, W, k, sparse, weights, flags, out, out_dtype, st);
+        run_blur_out<double>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
         break;
     default:
         throw Error("gaussian_blur: unsupported input dtype");

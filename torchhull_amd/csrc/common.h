@@ -168,9 +168,11 @@ struct SceneDev
     float corner[3];
     float length;
     // bitmap of grid vertices (R+1)^3 with rows padded to `row_words` 32-bit words
-    uint64_t bitmap_offset; // in words, into the batch's bitmap
+    uint64_t bitmap_offset; // in words, into the batch's vertex bitmap
     uint32_t row_words;
-    uint32_t pad_;
+    // bitmap of leaf cells R^3 with rows padded to `cell_row_words` words
+    uint32_t cell_row_words;
+    uint64_t cell_offset;   // in words, into the batch's cell bitmap
 };
 
 // ---------------------------------------------------------------------------

@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -22,9 +24,6 @@
 namespace thull
 {
 
-void launch_gaussian_blur_impl(const void* images, int32_t in_dtype, int64_t B, int64_t H, int64_t W, int k,
-                               bool sparse, const void* weights, void* out, int32_t out_dtype, uint8_t* flags,
-                               hipStream_t st);
 
 namespace
 {
@@ -169,6 +168,22 @@ struct Context
             ev_pool.push_back(e);
         }
         return ev_pool[i];
+    }
+
+    // device weight tables of the blur, keyed by (kernel size, sigma bits, output dtype, sparse)
+    struct BlurWeights
+    {
+        void* dev;
+        double norm;
+    };
+    std::map<std::tuple<int, uint32_t, int, int>, BlurWeights> blur_weights;
+
+    ~Context()
+    {
+        for (auto& kv : blur_weights)
+            (void)hipFree(kv.second.dev);
+        for (auto e : ev_pool)
+            (void)hipEventDestroy(e);
     }
 
     // last marching-cubes run
@@ -323,8 +338,9 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     ctx.faces = nullptr;
     std::vector<SceneDev> sd(n);
     int32_t max_level = 0;
-    uint64_t total_words = 0;
+    uint64_t total_words = 0, total_cell_words = 0;
     std::vector<uint64_t> word_begin(n + 1, 0);
+    std::vector<IntegralRequest> ireq;
     for (int32_t s = 0; s < n; ++s)
     {
         const thull_scene& sc = in[s];
@@ -349,22 +365,29 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         d.corner[1] = sc.cube_corner_bfl[1];
         d.corner[2] = sc.cube_corner_bfl[2];
         d.length = sc.cube_length;
-        const uint64_t R1 = (uint64_t(1) << sc.level) + 1;
+        const uint64_t R = uint64_t(1) << sc.level, R1 = R + 1;
         d.row_words = uint32_t((R1 + 31) / 32);
         d.bitmap_offset = total_words;
+        d.cell_row_words = uint32_t((R + 31) / 32);
+        d.cell_offset = total_cell_words;
         word_begin[s] = total_words;
         if (d.level >= 0)
+        {
             total_words += R1 * R1 * d.row_words;
+            total_cell_words += R * R * d.cell_row_words;
+        }
         max_level = std::max(max_level, d.level);
-        // integral image
         if (d.level >= 0)
         {
-            float* integral = pool.alloc<float>(uint64_t(sc.B) * sc.H * sc.W);
-            launch_integral_image(sc.masks, sc.mask_dtype, sc.B, sc.H, sc.W, integral, st);
+            float* integral = pool.alloc<float>(uint64_t(sc.B) * (sc.H + 1) * (sc.W + 1));
+            ireq.push_back(IntegralRequest{ sc.masks, sc.mask_dtype, sc.B, sc.H, sc.W, integral });
             d.integral = integral;
         }
         sd[s] = d;
     }
+    // integral images of every scene in one launch per pass
+    launch_integral_images_padded(ireq.data(), int32_t(ireq.size()),
+                                  pool.alloc_bytes(integral_job_bytes(std::max<size_t>(ireq.size(), 1))), st);
     word_begin[n] = total_words;
     ctx.scenes_host = sd;
     SceneDev* scenes = pool.alloc<SceneDev>(n);
@@ -378,13 +401,12 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         if (sd[s].level >= 0)
             live.push_back(s);
     uint64_t* list = pool.alloc<uint64_t>(std::max<size_t>(live.size(), 1));
-    {
-        std::vector<uint64_t> roots(live.size());
-        for (size_t k = 0; k < live.size(); ++k)
-            roots[k] = pack_key(uint32_t(live[k]), 0, 0, 0);
-        if (!roots.empty())
-            THULL_HIP(hipMemcpyAsync(list, roots.data(), roots.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-    }
+    std::vector<uint64_t> roots(live.size());
+    for (size_t k = 0; k < live.size(); ++k)
+        roots[k] = pack_key(uint32_t(live[k]), 0, 0, 0);
+    if (!roots.empty())
+        THULL_HIP(hipMemcpyAsync(list, roots.data(), roots.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    std::vector<uint8_t*> step_masks;
     uint32_t count = uint32_t(live.size());
     ctx.level_lists.push_back(list);
     ctx.level_sizes.push_back(count);
@@ -421,6 +443,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         check_u32(next, "octree level size");
         uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(next, 1));
         launch_carve_scatter(list, count, mask, boff, children, st);
+        step_masks.push_back(mask);
         list = children;
         count = uint32_t(next);
         ctx.level_lists.push_back(list);
@@ -428,25 +451,28 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     }
     const int32_t nlists = int32_t(ctx.level_lists.size());
 
-    // ---- corner vertices (VH:770-825) ----
-    uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
-    uint32_t* prefix = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
-    if (total_words > 0)
-        THULL_HIP(hipMemsetAsync(bitmap, 0, total_words * sizeof(uint32_t), st));
-    for (int32_t l = 0; l < nlists; ++l)
+    // ---- corner vertices (VH:770-825): leaf cells, then their corners by dilation ----
+    uint32_t* cells = pool.alloc<uint32_t>(std::max<uint64_t>(total_cell_words, 1));
+    if (total_cell_words > 0)
+        THULL_HIP(hipMemsetAsync(cells, 0, total_cell_words * sizeof(uint32_t), st));
+    launch_mark_root_cells(scenes, n, cells, st);
+    for (int32_t i = 0; i + 1 < nlists; ++i)
     {
         bool target = false;
         for (int32_t s = 0; s < n; ++s)
-            target |= sd[s].level == l;
+            target |= sd[s].level == i + 1;
         if (target)
-            launch_mark_corners(ctx.level_lists[l], ctx.level_sizes[l], l, scenes, bitmap, st);
+            launch_mark_leaf_cells(ctx.level_lists[i], ctx.level_sizes[i], step_masks[i], i + 1, scenes, cells, st);
     }
+    uint64_t* dev_word_begin = pool.alloc<uint64_t>(n + 1);
+    THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
+    uint32_t* prefix = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
+    launch_dilate_cells(cells, bitmap, total_words, dev_word_begin, scenes, n, st);
     uint32_t* wsums = pool.alloc<uint32_t>(scan_blocks(total_words) + 1);
     exclusive_scan(LoadPopcount{ bitmap }, total_words, prefix, wsums, dev_total, st);
     const uint64_t nverts = ctx.read_u64(dev_total, st);
     check_u32(nverts, "sparse field size");
-    uint64_t* dev_word_begin = pool.alloc<uint64_t>(n + 1);
-    THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     uint64_t* vertices = pool.alloc<uint64_t>(std::max<uint64_t>(nverts, 1));
     float* values = pool.alloc<float>(std::max<uint64_t>(nverts, 1));
     launch_extract_vertices(bitmap, prefix, total_words, dev_word_begin, scenes, n, vertices, st);
@@ -999,6 +1025,79 @@ gauss_weight(int i, int j, float sigma)
     return float(std::exp(double(arg)));
 }
 
+// Weight table in the output type and the weight sum of the band blur (the
+// reference's `normalization`, accumulated in tap order in the output
+// precision, GB:217-228); for the dense blur the table is normalised by the
+// float32 sum (GB:68) and the sum is unused.  Uploaded once per key.
+static Context::BlurWeights
+make_blur_weights(int k, float sigma, int32_t out_dtype, bool sparse)
+{
+    using namespace thull;
+    const int h = k / 2;
+    std::vector<float> w(size_t(k) * k);
+    for (int i = -h; i <= h; ++i)
+        for (int j = -h; j <= h; ++j)
+            w[(i + h) * k + (j + h)] = gauss_weight(i, j, sigma);
+    std::vector<unsigned char> table;
+    double norm = 0.0;
+    if (out_dtype == kF32)
+    {
+        std::vector<float> t(w);
+        float acc = 0.0f, fsum = 0.0f;
+        for (float v : w)
+        {
+            acc = acc + v;
+            fsum = fsum + v;
+        }
+        if (!sparse)
+            for (auto& v : t)
+                v = v / fsum;
+        norm = double(acc);
+        table.resize(t.size() * sizeof(float));
+        std::memcpy(table.data(), t.data(), table.size());
+    }
+    else if (out_dtype == kF64)
+    {
+        std::vector<double> t(w.size());
+        double acc = 0.0;
+        float fsum = 0.0f;
+        for (float v : w)
+        {
+            acc = acc + double(v);
+            fsum = fsum + v;
+        }
+        for (size_t q = 0; q < w.size(); ++q)
+            t[q] = sparse ? double(w[q]) : double(w[q]) / double(fsum);
+        norm = acc;
+        table.resize(t.size() * sizeof(double));
+        std::memcpy(table.data(), t.data(), table.size());
+    }
+    else
+    {
+        std::vector<__half> t(w.size());
+        __half acc = __float2half(0.0f);
+        float fsum = 0.0f;
+        for (float v : w)
+        {
+            const __half hv = __float2half(v);
+            acc = __float2half(__half2float(acc) + __half2float(hv));
+            fsum = fsum + __half2float(hv);
+        }
+        for (size_t q = 0; q < w.size(); ++q)
+        {
+            const __half hw = __float2half(w[q]);
+            t[q] = sparse ? hw : __float2half(__half2float(hw) / fsum);
+        }
+        norm = double(__half2float(acc));
+        table.resize(t.size() * sizeof(__half));
+        std::memcpy(table.data(), t.data(), table.size());
+    }
+    Context::BlurWeights bw{ nullptr, norm };
+    THULL_HIP(hipMalloc(&bw.dev, table.size()));
+    THULL_HIP(hipMemcpy(bw.dev, table.data(), table.size(), hipMemcpyHostToDevice));
+    return bw;
+}
+
 int
 thull_gaussian_blur(thull_ctx* ctx,
                     const void* images,
@@ -1024,62 +1123,70 @@ thull_gaussian_blur(thull_ctx* ctx,
         Context& c = ctx->c;
         hipStream_t st = static_cast<hipStream_t>(stream);
         c.begin(st);
-        const int k = kernel_size, h = k / 2;
-        std::vector<float> w(size_t(k) * k);
-        for (int i = -h; i <= h; ++i)
-            for (int j = -h; j <= h; ++j)
-                w[(i + h) * k + (j + h)] = gauss_weight(i, j, sigma);
+        const int k = kernel_size;
         const bool is_sparse = sparse != 0;
-        std::vector<unsigned char> table;
-        if (out_dtype == kF32)
+        uint32_t sbits;
+        std::memcpy(&sbits, &sigma, 4);
+        const auto key = std::make_tuple(k, sbits, int(out_dtype), int(is_sparse));
+        auto it = c.blur_weights.find(key);
+        if (it == c.blur_weights.end())
         {
-            std::vector<float> t(w);
-            if (!is_sparse)
-            {
-                float sum = 0.0f; // GB:68: kernel / float sum
-                for (float v : w)
-                    sum += v;
-                for (auto& v : t)
-                    v = v / sum;
-            }
-            table.resize(t.size() * sizeof(float));
-            std::memcpy(table.data(), t.data(), table.size());
-        }
-        else if (out_dtype == kF64)
-        {
-            std::vector<double> t(w.size());
-            float sum = 0.0f;
-            for (float v : w)
-                sum += v;
-            for (size_t q = 0; q < w.size(); ++q)
-                t[q] = is_sparse ? double(w[q]) : double(w[q]) / double(sum);
-            table.resize(t.size() * sizeof(double));
-            std::memcpy(table.data(), t.data(), table.size());
-        }
-        else
-        {
-            std::vector<__half> t(w.size());
-            float sum = 0.0f;
-            for (float v : w)
-                sum += __half2float(__float2half(v));
-            for (size_t q = 0; q < w.size(); ++q)
-            {
-                const __half hw = __float2half(w[q]);
-                t[q] = is_sparse ? hw : __float2half(__half2float(hw) / sum);
-            }
-            table.resize(t.size() * sizeof(__half));
-            std::memcpy(table.data(), t.data(), table.size());
+            Context::BlurWeights bw = make_blur_weights(k, sigma, out_dtype, is_sparse);
+            it = c.blur_weights.emplace(key, bw).first;
         }
         if (B == 0 || H == 0 || W == 0)
             return;
-        void* dweights = c.pool.alloc_bytes(table.size());
-        THULL_HIP(hipMemcpyAsync(dweights, table.data(), table.size(), hipMemcpyHostToDevice, st));
-        uint8_t* flags = nullptr;
+        BlurImage im{ images, out, nullptr, B, H, W };
         if (is_sparse && k > 1)
-            flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
-        launch_gaussian_blur_impl(images, in_dtype, B, H, W, k, is_sparse, dweights, out, out_dtype, flags, st);
-        // the weight table above is pageable host memory read by an async copy
-        THULL_HIP(hipStreamSynchronize(st));
+            im.flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
+        launch_gaussian_blur_batch(&im, 1, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype,
+                                   c.pool.alloc_bytes(blur_job_bytes(1)), st);
+    });
+}
+
+int
+thull_gaussian_blur_batch(thull_ctx* ctx,
+                          int32_t n,
+                          const void* const* images,
+                          int32_t in_dtype,
+                          const int64_t* sizes,
+                          int32_t kernel_size,
+                          float sigma,
+                          int32_t sparse,
+                          void* const* outs,
+                          int32_t out_dtype,
+                          void* stream)
+{
+    return guarded([&] {
+        using namespace thull;
+        THULL_CHECK(ctx != nullptr && images != nullptr && sizes != nullptr && outs != nullptr, "null argument");
+        THULL_CHECK(n >= 0, "n must be non-negative");
+        THULL_CHECK(kernel_size > 0, "kernel_size must be positive");
+        THULL_CHECK(kernel_size % 2 == 1, "kernel_size must be odd");
+        THULL_CHECK(sigma > 0.0f, "sigma must be positive");
+        THULL_CHECK(out_dtype == kF16 || out_dtype == kF32 || out_dtype == kF64, "dtype must be floating point");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        c.begin(st);
+        const int k = kernel_size;
+        const bool is_sparse = sparse != 0;
+        uint32_t sbits;
+        std::memcpy(&sbits, &sigma, 4);
+        const auto key = std::make_tuple(k, sbits, int(out_dtype), int(is_sparse));
+        auto it = c.blur_weights.find(key);
+        if (it == c.blur_weights.end())
+            it = c.blur_weights.emplace(key, make_blur_weights(k, sigma, out_dtype, is_sparse)).first;
+        std::vector<BlurImage> ims(n);
+        for (int32_t i = 0; i < n; ++i)
+        {
+            const int64_t B = sizes[3 * i], H = sizes[3 * i + 1], W = sizes[3 * i + 2];
+            THULL_CHECK(B >= 0 && H >= 0 && W >= 0, "sizes must be non-negative");
+            ims[i] = BlurImage{ images[i], outs[i], nullptr, B, H, W };
+            if (is_sparse && k > 1)
+                ims[i].flags = c.pool.alloc<uint8_t>(std::max<uint64_t>(1, blur_tile_flag_bytes(B, H, W, k)));
+        }
+        launch_gaussian_blur_batch(ims.data(), n, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype,
+                                   c.pool.alloc_bytes(blur_job_bytes(std::max(n, 1))), st);
     });
 }
 
@@ -1090,8 +1197,10 @@ thull_integral_image(thull_ctx* ctx, const void* masks, int32_t dtype, int64_t B
     return guarded([&] {
         THULL_CHECK(ctx != nullptr, "ctx is null");
         THULL_CHECK(H * W < (int64_t(1) << 31), "H * W must be below 2^31");
-        THULL_HIP(hipSetDevice(ctx->c.device));
-        thull::launch_integral_image(masks, dtype, B, H, W, out, static_cast<hipStream_t>(stream));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        ctx->c.begin(st);
+        thull::launch_integral_image(masks, dtype, B, H, W, out, ctx->c.pool.alloc_bytes(thull::integral_job_bytes(1)),
+                                     st);
     });
 }
 

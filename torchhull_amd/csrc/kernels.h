@@ -9,8 +9,22 @@ namespace thull
 // ---- integral image (vh_kernels.hip) ----
 // float32 integral image of masks [B,H,W] of element type `dtype`, equal to
 // torch::cumsum(.., 1, f32) followed by cumsum(.., 2, f32) on CUDA.
-void launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out,
+// `job_buf` is device scratch of integral_job_bytes(number of tensors) bytes.
+size_t integral_job_bytes(size_t njobs);
+void launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, void* job_buf,
                            hipStream_t st);
+// The same integral images, for many mask tensors in one launch, in the layout
+// carving reads: per camera a (H+1) x (W+1) plane whose row 0 and column 0 are
+// zero, so I(y, x) sits at (y+1)*(W+1) + (x+1) and the zero padding of
+// image_utils.h:84-97 needs no branch.
+struct IntegralRequest
+{
+    const void* masks;
+    int32_t dtype;
+    int64_t B, H, W;
+    float* padded;
+};
+void launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_buf, hipStream_t st);
 
 // ---- octree carving (vh_kernels.hip) ----
 constexpr int kCarveThreads = 256;
@@ -26,10 +40,15 @@ void launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level_i,
 void launch_carve_scatter(const uint64_t* parents, uint32_t n, const uint8_t* child_mask,
                           const uint32_t* block_offsets, uint64_t* children, hipStream_t st);
 
-// Marks the 8 corner vertices of every cell of `cells` whose scene has
-// target level `level` in the scene's vertex bitmap.
-void launch_mark_corners(const uint64_t* cells, uint32_t n, int32_t level, const SceneDev* scenes, uint32_t* bitmap,
-                         hipStream_t st);
+// Sets the leaf-cell bits of the scenes whose target level is `level` from
+// the parents and child masks of carving step level - 1.
+void launch_mark_leaf_cells(const uint64_t* parents, uint32_t n, const uint8_t* child_mask, int32_t level,
+                            const SceneDev* scenes, uint32_t* cells, hipStream_t st);
+// Sets the root cell of every level-0 scene.
+void launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells, hipStream_t st);
+// Vertex bitmap (all words, no clearing needed) = leaf cells dilated by {0,1}^3.
+void launch_dilate_cells(const uint32_t* cells, uint32_t* verts, uint64_t nwords, const uint64_t* scene_word_begin,
+                         const SceneDev* scenes, int32_t nscenes, hipStream_t st);
 
 // Writes the packed key of every set bit of the bitmap at prefix[word] + rank.
 // `scene_word_begin` has nscenes + 1 entries (word offsets of each scene).
@@ -98,9 +117,16 @@ void launch_extract_grid_vertices(const uint32_t* bitmap, const uint32_t* prefix
                                   uint64_t* vertices, hipStream_t st);
 
 // ---- gaussian blur (blur_kernels.hip) ----
-void launch_gaussian_blur(const void* images, int32_t in_dtype, int64_t B, int64_t H, int64_t W, int kernel_size,
-                          float sigma, bool sparse, void* out, int32_t out_dtype, uint8_t* tile_flags,
-                          hipStream_t st);
+struct BlurImage
+{
+    const void* in;  // [B,H,W]
+    void* out;       // [B,H,W] of the output dtype
+    uint8_t* flags;  // blur_tile_flag_bytes(B, H, W, k) bytes (sparse, k > 1)
+    int64_t B, H, W;
+};
 int64_t blur_tile_flag_bytes(int64_t B, int64_t H, int64_t W, int kernel_size);
+size_t blur_job_bytes(size_t n);
+void launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int kernel_size, bool sparse,
+                                const void* weights, double norm, int32_t out_dtype, void* job_buf, hipStream_t st);
 
 } // namespace thull

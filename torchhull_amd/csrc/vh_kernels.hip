@@ -7,6 +7,7 @@
 // and VH:827-888.
 
 #include <cfloat>
+#include <vector>
 
 #include <hip/hip_fp16.h>
 
@@ -57,16 +58,41 @@ mask_at(const void* masks, int32_t dtype, int64_t idx)
 // chains.
 // ===========================================================================
 
-template <typename T>
-__global__ __launch_bounds__(256) void
-integral_cols_kernel(const T* __restrict__ m, float* __restrict__ out, int32_t H, int32_t W)
+// One integral-image job: B cameras of one mask tensor.  Blocks of every job
+// are flattened into one launch; `block_begin` is the job's first block.
+struct IntegralJob
 {
-    const int32_t x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= W)
-        return;
-    const int64_t plane = int64_t(H) * W;
-    const T* src = m + blockIdx.y * plane + x;
-    float* dst = out + blockIdx.y * plane + x;
+    const void* masks;   // [B,H,W]
+    float* out;          // element (b, y, x) at out + b * plane + y * ld + x
+    int64_t plane;
+    int32_t ld;
+    int32_t dtype;
+    int32_t B, H, W;
+    uint32_t col_blocks;  // blocks per camera of the column pass
+    uint32_t row_blocks;  // blocks per camera of the row pass
+    uint64_t col_begin;   // first block of this job in the column pass
+    uint64_t row_begin;   // first block of this job in the row pass
+};
+
+__device__ __forceinline__ int32_t
+find_job(const IntegralJob* jobs, int32_t n, uint64_t blk, bool rows)
+{
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if ((rows ? jobs[mid].row_begin : jobs[mid].col_begin) <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ void
+integral_cols(const T* __restrict__ src, float* __restrict__ dst, int32_t H, int32_t W, int32_t ld)
+{
     float acc = 0.0f;
     int32_t y = 0;
     constexpr int U = 16;
@@ -80,34 +106,58 @@ integral_cols_kernel(const T* __restrict__ m, float* __restrict__ out, int32_t H
         for (int k = 0; k < U; ++k)
         {
             acc = acc + v[k];
-            dst[int64_t(y + k) * W] = acc;
+            dst[int64_t(y + k) * ld] = acc;
         }
     }
     for (; y < H; ++y)
     {
         acc = acc + as_float(src[int64_t(y) * W]);
-        dst[int64_t(y) * W] = acc;
+        dst[int64_t(y) * ld] = acc;
     }
+}
+
+// Column pass: one thread per column (coalesced across the wave), sequential over rows.
+__global__ __launch_bounds__(256) void
+integral_cols_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
+{
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, false)];
+    const uint64_t local = blk - j.col_begin;
+    const int32_t b = int32_t(local / j.col_blocks);
+    const int32_t x = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    if (x >= j.W)
+        return;
+    const int64_t in_off = int64_t(b) * j.H * j.W + x;
+    float* dst = j.out + b * j.plane + x;
+    if (j.dtype == kF32)
+        integral_cols(static_cast<const float*>(j.masks) + in_off, dst, j.H, j.W, j.ld);
+    else
+        integral_cols(static_cast<const uint8_t*>(j.masks) + in_off, dst, j.H, j.W, j.ld);
 }
 
 // Row pass in place: one wavefront owns 64 rows and walks 64x64 tiles through
 // LDS, so loads and stores stay coalesced while every lane runs the
 // sequential chain of its own row.
 __global__ __launch_bounds__(64) void
-integral_rows_kernel(float* __restrict__ io, int32_t H, int32_t W)
+integral_rows_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
 {
     __shared__ float tile[64][65];
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, true)];
+    const uint64_t local = blk - j.row_begin;
+    const int32_t b = int32_t(local / j.row_blocks);
+    const int32_t r0 = int32_t(local - uint64_t(b) * j.row_blocks) * 64;
     const int lane = threadIdx.x;
-    const int32_t r0 = blockIdx.x * 64;
-    float* img = io + int64_t(blockIdx.y) * H * W;
-    const int32_t rows = min(64, H - r0);
+    float* img = j.out + b * j.plane;
+    const int32_t rows = min(64, j.H - r0);
+    const int32_t W = j.W, ld = j.ld;
     float acc = 0.0f;
     for (int32_t c0 = 0; c0 < W; c0 += 64)
     {
         const int32_t cols = min(64, W - c0);
         if (lane < cols)
             for (int r = 0; r < rows; ++r)
-                tile[r][lane] = img[int64_t(r0 + r) * W + c0 + lane];
+                tile[r][lane] = img[int64_t(r0 + r) * ld + c0 + lane];
         __syncthreads();
         if (lane < rows)
             for (int c = 0; c < cols; ++c)
@@ -118,34 +168,95 @@ integral_rows_kernel(float* __restrict__ io, int32_t H, int32_t W)
         __syncthreads();
         if (lane < cols)
             for (int r = 0; r < rows; ++r)
-                img[int64_t(r0 + r) * W + c0 + lane] = tile[r][lane];
+                img[int64_t(r0 + r) * ld + c0 + lane] = tile[r][lane];
         __syncthreads();
     }
 }
 
+// Zero row and zero column of the padded layout used by carving (one block per camera and 256 entries).
+__global__ __launch_bounds__(256) void
+integral_border_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
+{
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, false)];
+    const uint64_t local = blk - j.col_begin;
+    const int32_t b = int32_t(local / j.col_blocks);
+    const int32_t i = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    // padded plane base: out points at element (0, 0), one row and one column into the plane
+    float* p = j.out - j.ld - 1 + b * j.plane;
+    for (int32_t q = i; q <= max(j.H, j.W); q += int32_t(j.col_blocks) * 256)
+    {
+        if (q <= j.W)
+            p[q] = 0.0f;
+        if (q <= j.H)
+            p[int64_t(q) * j.ld] = 0.0f;
+    }
+}
+
+static void
+integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, bool border, hipStream_t st)
+{
+    uint64_t cols = 0, rows = 0;
+    for (auto& j : jobs)
+    {
+        j.col_blocks = blocks_for(j.W, 256);
+        j.row_blocks = blocks_for(j.H, 64);
+        j.col_begin = cols;
+        j.row_begin = rows;
+        cols += uint64_t(j.col_blocks) * j.B;
+        rows += uint64_t(j.row_blocks) * j.B;
+    }
+    if (cols == 0)
+        return;
+    THULL_CHECK(cols < (uint64_t(1) << 31) && rows < (uint64_t(1) << 31), "integral image batch too large");
+    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(IntegralJob), hipMemcpyHostToDevice, st));
+    const int32_t n = int32_t(jobs.size());
+    if (border)
+    {
+        integral_border_kernel<<<unsigned(cols), 256, 0, st>>>(djobs, n);
+        THULL_LAUNCH_CHECK();
+    }
+    integral_cols_kernel<<<unsigned(cols), 256, 0, st>>>(djobs, n);
+    THULL_LAUNCH_CHECK();
+    integral_rows_kernel<<<unsigned(rows), 64, 0, st>>>(djobs, n);
+    THULL_LAUNCH_CHECK();
+    // the job table is pageable host memory read by an async copy
+    THULL_HIP(hipStreamSynchronize(st));
+}
+
+size_t
+integral_job_bytes(size_t njobs)
+{
+    return njobs * sizeof(IntegralJob);
+}
+
 void
-launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, hipStream_t st)
+launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, void* job_buf,
+                      hipStream_t st)
 {
     if (B == 0 || H == 0 || W == 0)
         return;
-    dim3 gc(blocks_for(W, 256), unsigned(B));
-    switch (dtype)
+    THULL_CHECK(dtype == kF32 || dtype == kU8 || dtype == kBool, "integral image: unsupported mask element type");
+    std::vector<IntegralJob> jobs(1);
+    jobs[0] = IntegralJob{ masks, out, H * W, int32_t(W), dtype, int32_t(B), int32_t(H), int32_t(W), 0, 0, 0, 0 };
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), false, st);
+}
+
+void
+launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_buf, hipStream_t st)
+{
+    std::vector<IntegralJob> jobs;
+    for (int32_t i = 0; i < n; ++i)
     {
-    case kF32:
-        integral_cols_kernel<float><<<gc, 256, 0, st>>>(static_cast<const float*>(masks), out, int32_t(H), int32_t(W));
-        break;
-    case kU8:
-    case kBool:
-        integral_cols_kernel<uint8_t>
-                <<<gc, 256, 0, st>>>(static_cast<const uint8_t*>(masks), out, int32_t(H), int32_t(W));
-        break;
-    default:
-        throw Error("integral image: unsupported mask element type");
+        const IntegralRequest& r = req[i];
+        if (r.B == 0 || r.H == 0 || r.W == 0)
+            continue;
+        THULL_CHECK(r.dtype == kF32 || r.dtype == kU8 || r.dtype == kBool,
+                    "integral image: unsupported mask element type");
+        jobs.push_back(IntegralJob{ r.masks, r.padded + (r.W + 1) + 1, (r.H + 1) * (r.W + 1), int32_t(r.W + 1),
+                                    r.dtype, int32_t(r.B), int32_t(r.H), int32_t(r.W), 0, 0, 0, 0 });
     }
-    THULL_LAUNCH_CHECK();
-    dim3 gr(blocks_for(H, 64), unsigned(B));
-    integral_rows_kernel<<<gr, 64, 0, st>>>(out, int32_t(H), int32_t(W));
-    THULL_LAUNCH_CHECK();
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), true, st);
 }
 
 // ===========================================================================
@@ -196,6 +307,11 @@ load_camera(const float* __restrict__ T, bool gl)
     return c;
 }
 
+// Pixel box of a set of projected corners.  The reference starts the box at
+// (FLT_MAX, -FLT_MAX) and folds the corners in with fminf / fmaxf, which
+// ignore NaN (VH:201-234); folding FLT_MAX into every minimum keeps an
+// all-NaN minimum at FLT_MAX.  The maximum needs no -FLT_MAX fold in full
+// mode: an all-NaN maximum and -FLT_MAX both clamp to pixel 0 after + 0.5.
 struct Box
 {
     float minx, miny, maxx, maxy;
@@ -204,78 +320,202 @@ struct Box
 __device__ __forceinline__ Box
 box_merge(const Box& a, const Box& b)
 {
-    return Box{ fminf(a.minx, b.minx), fminf(a.miny, b.miny), fmaxf(a.maxx, b.maxx), fmaxf(a.maxy, b.maxy) };
+    return Box{ fminf(fminf(a.minx, b.minx), FLT_MAX), fminf(fminf(a.miny, b.miny), FLT_MAX), fmaxf(a.maxx, b.maxx),
+                fmaxf(a.maxy, b.maxy) };
 }
 
-// Classification of one child against one camera.  `bit` is the child's bit.
-// Returns nothing; updates alive / refine / inside masks.
-__device__ __forceinline__ void
-classify_child_camera(Box bb,
-                      bool partial,
-                      bool fully_inside,
-                      const float* __restrict__ I,
-                      int32_t H,
-                      int32_t W,
-                      float Hm1,
-                      float Wm1,
-                      uint32_t bit,
-                      uint32_t& alive,
-                      uint32_t& refine,
-                      uint32_t& inside_any)
+// roundf (halves away from zero) of a value known to be >= 0.
+__device__ __forceinline__ float
+round_nonneg(float v)
 {
-    // The reference starts the box at (FLT_MAX, -FLT_MAX) and folds the 8
-    // corners in with fminf/fmaxf (VH:201-234); NaN corners are ignored.
-    bb.minx = fminf(FLT_MAX, bb.minx);
-    bb.miny = fminf(FLT_MAX, bb.miny);
-    bb.maxx = fmaxf(-FLT_MAX, bb.maxx);
-    bb.maxy = fmaxf(-FLT_MAX, bb.maxy);
-    // VH:236-245
-    const float rminx = roundf(bb.minx + -0.5f);
-    const float rminy = roundf(bb.miny + -0.5f);
-    const float rmaxx = roundf(bb.maxx + 0.5f);
-    const float rmaxy = roundf(bb.maxy + 0.5f);
-    const int32_t x0 = clamp_rounded(rminx, Wm1);
-    const int32_t y0 = clamp_rounded(rminy, Hm1);
-    const int32_t x1 = clamp_rounded(rmaxx, Wm1);
-    const int32_t y1 = clamp_rounded(rmaxy, Hm1);
-    const int32_t area = (y1 - y0 + 1) * (x1 - x0 + 1); // VH:247
-    // VH:249-257, zero padding outside the image
-    const float i11 = I[y1 * W + x1];
-    const float i10 = (x0 > 0) ? I[y1 * W + (x0 - 1)] : 0.0f;
-    const float i01 = (y0 > 0) ? I[(y0 - 1) * W + x1] : 0.0f;
-    const float i00 = (x0 > 0 && y0 > 0) ? I[(y0 - 1) * W + (x0 - 1)] : 0.0f;
-    const float sum = ((i11 + i00) - i10) - i01;
-    if (!partial)
+    const float t = truncf(v);
+    return (v - t >= 0.5f) ? t + 1.0f : t;
+}
+
+// clamp(int64(roundf(v)), 0, hi) for the full-mask test: the clamp bounds are
+// integers and roundf is monotone, so clamping first gives the same integer,
+// NaN included (both map to 0).
+__device__ __forceinline__ int32_t
+clamp_round(float v, float hi)
+{
+    return int32_t(round_nonneg(fminf(fmaxf(v, 0.0f), hi)));
+}
+
+// One child against one camera (VH:236-280 and VH:365-412).  I is the camera's
+// integral image in the zero-padded layout, pitch ld = W + 1.  Returns true
+// when the camera classifies the child as empty.
+template <bool PARTIAL>
+__device__ __forceinline__ bool
+child_camera(const Box& bb, bool fully_inside, const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1,
+             uint32_t bit, uint32_t& refine, uint32_t& inside_any)
+{
+    int32_t x0, y0, x1, y1;
+    bool same = true;
+    if constexpr (!PARTIAL)
     {
-        // VH:269-280
-        if (sum <= kMargin)
-            alive &= ~bit;
-        else if (!(sum >= float(area) - kMargin))
-            refine |= bit;
+        x0 = clamp_round(bb.minx + -0.5f, Wm1);
+        y0 = clamp_round(bb.miny + -0.5f, Hm1);
+        x1 = clamp_round(bb.maxx + 0.5f, Wm1);
+        y1 = clamp_round(bb.maxy + 0.5f, Hm1);
     }
     else
     {
-        // VH:376-412; the unclamped box area uses int64 wrap-around like the GPU reference
-        const uint64_t fy = uint64_t(f2i64_sat(rmaxy)) - uint64_t(f2i64_sat(rminy)) + 1u;
-        const uint64_t fx = uint64_t(f2i64_sat(rmaxx)) - uint64_t(f2i64_sat(rminx)) + 1u;
-        const bool same = int64_t(fy * fx) == int64_t(area);
-        if (sum <= kMargin && same)
-            alive &= ~bit;
-        else if (sum >= float(area) - kMargin && same)
+        const float rx0 = roundf(bb.minx + -0.5f);
+        const float ry0 = roundf(bb.miny + -0.5f);
+        const float rx1 = roundf(fmaxf(bb.maxx, -FLT_MAX) + 0.5f);
+        const float ry1 = roundf(fmaxf(bb.maxy, -FLT_MAX) + 0.5f);
+        const float fx0 = fminf(fmaxf(rx0, 0.0f), Wm1), fy0 = fminf(fmaxf(ry0, 0.0f), Hm1);
+        const float fx1 = fminf(fmaxf(rx1, 0.0f), Wm1), fy1 = fminf(fmaxf(ry1, 0.0f), Hm1);
+        x0 = int32_t(fx0);
+        y0 = int32_t(fy0);
+        x1 = int32_t(fx1);
+        y1 = int32_t(fy1);
+        if (!(rx0 == fx0 && rx1 == fx1 && ry0 == fy0 && ry1 == fy1))
         {
+            // clamped somewhere: compare with the unclamped box area in int64 (wrap-around like the GPU reference)
+            const uint64_t fy = uint64_t(f2i64_sat(ry1)) - uint64_t(f2i64_sat(ry0)) + 1u;
+            const uint64_t fx = uint64_t(f2i64_sat(rx1)) - uint64_t(f2i64_sat(rx0)) + 1u;
+            same = int64_t(fy * fx) == int64_t(y1 - y0 + 1) * int64_t(x1 - x0 + 1);
         }
-        else if (y1 - y0 > 1 && x1 - x0 > 1)
+    }
+    // VH:249-257: four reads of the integral image; zero padding by the zero row / column
+    const uint32_t r0 = uint32_t(y0) * ld, r1 = uint32_t(y1 + 1) * ld;
+    const float i11 = I[r1 + uint32_t(x1) + 1u];
+    const float i00 = I[r0 + uint32_t(x0)];
+    const float i10 = I[r1 + uint32_t(x0)];
+    const float i01 = I[r0 + uint32_t(x1) + 1u];
+    const float sum = ((i11 + i00) - i10) - i01;
+    const float area = float(y1 - y0 + 1) * float(x1 - x0 + 1); // == float(int64 area): one rounding of the product
+    if constexpr (!PARTIAL)
+    {
+        if (sum <= kMargin)
+            return true;
+        if (!(sum >= area - kMargin))
             refine |= bit;
+        return false;
+    }
+    else
+    {
         if (fully_inside)
             inside_any |= bit;
+        if (sum <= kMargin && same)
+            return true;
+        if (sum >= area - kMargin && same)
+            return false;
+        if (y1 - y0 > 1 && x1 - x0 > 1)
+            refine |= bit;
+        return false;
     }
 }
 
-__device__ uint32_t
-classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_t li, bool last, uint32_t& tests)
+// Projects the 3x3x3 lattice of one parent for one camera and classifies the
+// live children.  `dead` collects children found empty by this camera.
+template <bool PARTIAL>
+__device__ __forceinline__ uint32_t
+parent_camera(const SceneDev& s,
+              const float (&wx)[3],
+              const float (&wy)[3],
+              const float (&wz)[3],
+              int32_t b,
+              uint32_t alive,
+              uint32_t& refine,
+              uint32_t& inside_any)
+{
+    const bool gl = s.opengl != 0;
+    const float fH = float(s.H), fW = float(s.W);
+    const float Hm1 = float(s.H - 1), Wm1 = float(s.W - 1);
+    // VH:358-362 with roundf folded into the comparison: 1 <= roundf(p) <= n-2  <=>  0.5 <= p < n-1.5
+    const float Wlim = float(s.W) - 1.5f, Hlim = float(s.H) - 1.5f;
+    const uint32_t ld = uint32_t(s.W) + 1u;
+    const CarveCamera cam = load_camera(s.transforms + 16 * b, gl);
+    const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld;
+
+    Box win[3][4];
+    uint32_t inbits[3] = { 0u, 0u, 0u };
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+    {
+        const float ax = fmaf(cam.x2, wz[w], cam.x3);
+        const float ay = fmaf(cam.y2, wz[w], cam.y3);
+        const float ad = fmaf(cam.d2, wz[w], cam.d3);
+        float ppx[3][3], ppy[3][3]; // [v][u]
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+        {
+            const float bx = fmaf(cam.x1, wy[v], ax);
+            const float by = fmaf(cam.y1, wy[v], ay);
+            const float bd = fmaf(cam.d1, wy[v], ad);
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+            {
+                const float cx = fmaf(cam.x0, wx[u], bx);
+                const float cy = fmaf(cam.y0, wx[u], by);
+                const float cd = fmaf(cam.d0, wx[u], bd);
+                const float qx = cx / cd;
+                const float qy = cy / cd;
+                float pxl, pyl;
+                if (gl)
+                {
+                    // image_utils.h:67-71; t * 0.5 is exact, so fmaf(t, 0.5, -0.5) == (t / 2) - 0.5
+                    pxl = fmaf((qx + 1.0f) * fW, 0.5f, -0.5f);
+                    pyl = fmaf((qy + 1.0f) * fH, 0.5f, -0.5f);
+                }
+                else
+                {
+                    pxl = qx - 0.5f;
+                    pyl = qy - 0.5f;
+                }
+                ppx[v][u] = pxl;
+                ppy[v][u] = pyl;
+                if constexpr (PARTIAL)
+                {
+                    if (pxl >= 0.5f && pxl < Wlim && pyl >= 0.5f && pyl < Hlim)
+                        inbits[w] |= 1u << (u + 3 * v);
+                }
+            }
+        }
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+            {
+                const float mnx0 = fminf(ppx[bb][a], ppx[bb][a + 1]);
+                const float mnx1 = fminf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
+                const float mny0 = fminf(ppy[bb][a], ppy[bb][a + 1]);
+                const float mny1 = fminf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
+                const float mxx0 = fmaxf(ppx[bb][a], ppx[bb][a + 1]);
+                const float mxx1 = fmaxf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
+                const float mxy0 = fmaxf(ppy[bb][a], ppy[bb][a + 1]);
+                const float mxy1 = fmaxf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
+                win[w][a + 2 * bb] = Box{ fminf(mnx0, mnx1), fminf(mny0, mny1), fmaxf(mxx0, mxx1), fmaxf(mxy0, mxy1) };
+            }
+    }
+    uint32_t dead = 0u;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const uint32_t bit = 1u << c;
+        if (!(alive & bit))
+            continue;
+        const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
+        const Box box = box_merge(win[cz][a + 2 * bb], win[cz + 1][a + 2 * bb]);
+        bool inside = false;
+        if constexpr (PARTIAL)
+        {
+            const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * bb);
+            inside = ((inbits[cz] & mm) == mm) && ((inbits[cz + 1] & mm) == mm);
+        }
+        if (child_camera<PARTIAL>(box, inside, I, ld, Hm1, Wm1, bit, refine, inside_any))
+            dead |= bit;
+    }
+    return dead;
+}
+
+__device__ __forceinline__ void
+lattice_world(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_t li, float (&wx)[3], float (&wy)[3],
+              float (&wz)[3])
 {
     const float inv = __builtin_ldexpf(1.0f, -(li + 1)); // 1 / resolution_children, exact
-    float wx[3], wy[3], wz[3];
 #pragma unroll
     for (int u = 0; u < 3; ++u)
     {
@@ -283,108 +523,36 @@ classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_
         wy[u] = grid_to_world(2 * py + u, inv, s.length, s.corner[1]);
         wz[u] = grid_to_world(2 * pz + u, inv, s.length, s.corner[2]);
     }
-    const bool gl = s.opengl != 0;
-    const bool partial = s.partial != 0;
-    const int32_t H = s.H, W = s.W;
-    const float fH = float(H), fW = float(W);
-    const float Hm1 = float(H - 1), Wm1 = float(W - 1);
-    const float Hm2 = float(H - 2), Wm2 = float(W - 2);
-    const int64_t plane = int64_t(H) * W;
+}
 
+// All cameras in index order for one parent; returns the occupancy mask.
+template <bool PARTIAL>
+__device__ uint32_t
+classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_t li, bool last, uint32_t& tests)
+{
+    float wx[3], wy[3], wz[3];
+    lattice_world(s, px, py, pz, li, wx, wy, wz);
     uint32_t alive = 0xffu, refine = 0u, inside_any = 0u;
-    for (int32_t b = 0; b < s.B && alive != 0u; ++b)
+    for (int32_t b = 0; b < s.B; ++b)
     {
+        if (alive == 0u)
+            break;
         tests += __popc(alive);
-        const CarveCamera cam = load_camera(s.transforms + 16 * b, gl);
-        const float* I = s.integral + b * plane;
-
-        // window boxes of the three z layers: win[w][a + 2 b] covers lattice
-        // points (a..a+1, b..b+1, w)
-        Box win[3][4];
-        uint32_t inbits[3] = { 0u, 0u, 0u }; // per layer, bit u + 3 v: rounded pixel inside with margin 1
-#pragma unroll
-        for (int w = 0; w < 3; ++w)
-        {
-            const float ax = fmaf(cam.x2, wz[w], cam.x3);
-            const float ay = fmaf(cam.y2, wz[w], cam.y3);
-            const float ad = fmaf(cam.d2, wz[w], cam.d3);
-            float ppx[3][3], ppy[3][3]; // [v][u]
-#pragma unroll
-            for (int v = 0; v < 3; ++v)
-            {
-                const float bx = fmaf(cam.x1, wy[v], ax);
-                const float by = fmaf(cam.y1, wy[v], ay);
-                const float bd = fmaf(cam.d1, wy[v], ad);
-#pragma unroll
-                for (int u = 0; u < 3; ++u)
-                {
-                    const float cx = fmaf(cam.x0, wx[u], bx);
-                    const float cy = fmaf(cam.y0, wx[u], by);
-                    const float cd = fmaf(cam.d0, wx[u], bd);
-                    const float qx = cx / cd;
-                    const float qy = cy / cd;
-                    float pxl, pyl;
-                    if (gl)
-                    {
-                        pxl = unnormalize_ndc(qx, fW);
-                        pyl = unnormalize_ndc(qy, fH);
-                    }
-                    else
-                    {
-                        pxl = qx - 0.5f;
-                        pyl = qy - 0.5f;
-                    }
-                    ppx[v][u] = pxl;
-                    ppy[v][u] = pyl;
-                    if (partial)
-                    {
-                        // VH:358-362: roundf(pixel) inside the image with margin 1
-                        const float rx = roundf(pxl), ry = roundf(pyl);
-                        if (rx >= 1.0f && rx <= Wm2 && ry >= 1.0f && ry <= Hm2)
-                            inbits[w] |= 1u << (u + 3 * v);
-                    }
-                }
-            }
-            // 2x2 windows in the layer: reduce along u, then along v
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-                {
-                    const float mnx0 = fminf(ppx[bb][a], ppx[bb][a + 1]);
-                    const float mnx1 = fminf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
-                    const float mny0 = fminf(ppy[bb][a], ppy[bb][a + 1]);
-                    const float mny1 = fminf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
-                    const float mxx0 = fmaxf(ppx[bb][a], ppx[bb][a + 1]);
-                    const float mxx1 = fmaxf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
-                    const float mxy0 = fmaxf(ppy[bb][a], ppy[bb][a + 1]);
-                    const float mxy1 = fmaxf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
-                    win[w][a + 2 * bb] = Box{ fminf(mnx0, mnx1), fminf(mny0, mny1), fmaxf(mxx0, mxx1), fmaxf(mxy0, mxy1) };
-                }
-        }
-
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            const uint32_t bit = 1u << c;
-            if (!(alive & bit))
-                continue;
-            const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const Box box = box_merge(win[cz][a + 2 * bb], win[cz + 1][a + 2 * bb]);
-            bool inside = false;
-            if (partial)
-            {
-                const uint32_t m = (3u << a) | (3u << (a + 3));
-                const uint32_t mm = m << (3 * bb);
-                inside = ((inbits[cz] & mm) == mm) && ((inbits[cz + 1] & mm) == mm);
-            }
-            classify_child_camera(box, partial, inside, I, H, W, Hm1, Wm1, bit, alive, refine, inside_any);
-        }
+        alive &= ~parent_camera<PARTIAL>(s, wx, wy, wz, b, alive, refine, inside_any);
     }
     uint32_t occ = refine & alive; // should_refine && !is_empty
-    if (partial && last)
+    if (PARTIAL && last)
         occ &= inside_any; // (!last_children || fully_inside_one_frame)
     return occ;
+}
+
+template <bool PARTIAL>
+__device__ __forceinline__ uint32_t
+classify_entry(const SceneDev& s, uint64_t key, int32_t level_i, uint32_t& tests)
+{
+    if (level_i >= s.level)
+        return 0u;
+    return classify_parent<PARTIAL>(s, key_x(key), key_y(key), key_z(key), level_i, level_i == s.level - 1, tests);
 }
 
 __global__ __launch_bounds__(kCarveThreads) void
@@ -402,9 +570,25 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
     if (t < n)
     {
         const uint64_t key = parents[t];
-        const SceneDev s = scenes[key_scene(key)];
-        if (level_i < s.level)
-            mask = classify_parent(s, key_x(key), key_y(key), key_z(key), level_i, level_i == s.level - 1, tests);
+        const uint32_t sid = key_scene(key);
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+        if (__all(sid == s0))
+        {
+            // the whole wave works on one scene: camera data and integral bases are wave-uniform
+            const SceneDev s = scenes[s0];
+            if (s.partial)
+                mask = classify_entry<true>(s, key, level_i, tests);
+            else
+                mask = classify_entry<false>(s, key, level_i, tests);
+        }
+        else
+        {
+            const SceneDev s = scenes[sid];
+            if (s.partial)
+                mask = classify_entry<true>(s, key, level_i, tests);
+            else
+                mask = classify_entry<false>(s, key, level_i, tests);
+        }
         child_mask[t] = uint8_t(mask);
     }
     uint32_t total;
@@ -415,6 +599,113 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
     block_exclusive_sum(tests, lds, tsum);
     if (threadIdx.x == 0 && tests_out != nullptr)
         atomicAdd(tests_out, (unsigned long long)tsum);
+}
+
+// Small levels are latency-bound with one thread per parent (a chain of up to
+// B dependent camera iterations).  Here G lanes share a parent and take the
+// cameras b = j, j + G, ...; the result flags are ORs over cameras and a
+// child is empty if any camera says so, so the lanes combine exactly by
+// OR / min across the group.  The canonical early-exit count is rebuilt from
+// the first camera that found each child empty.
+template <int G, bool PARTIAL>
+__device__ __forceinline__ void
+classify_split(const SceneDev& s, uint64_t key, int32_t level_i, int32_t j, uint32_t& refine, uint32_t& inside,
+               uint32_t (&first_empty)[8])
+{
+    float wx[3], wy[3], wz[3];
+    lattice_world(s, key_x(key), key_y(key), key_z(key), level_i, wx, wy, wz);
+    uint32_t alive = 0xffu;
+    for (int32_t b = j; b < s.B && alive != 0u; b += G)
+    {
+        const uint32_t dead = parent_camera<PARTIAL>(s, wx, wy, wz, b, alive, refine, inside);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (dead & (1u << c))
+                first_empty[c] = uint32_t(b);
+        alive &= ~dead;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void
+carve_classify_split_kernel(const uint64_t* __restrict__ parents,
+                            uint32_t n,
+                            int32_t level_i,
+                            const SceneDev* __restrict__ scenes,
+                            uint8_t* __restrict__ child_mask,
+                            unsigned long long* __restrict__ tests_out)
+{
+    __shared__ uint32_t lds[256 / 64];
+    const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t p = gt / G;
+    const int32_t j = int32_t(gt % G);
+    uint32_t refine = 0u, inside = 0u;
+    uint32_t first_empty[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        first_empty[c] = 0xffffffffu;
+    bool active = false, partial = false, last = false;
+    int32_t B = 0;
+    if (p < n)
+    {
+        const uint64_t key = parents[p];
+        const SceneDev s = scenes[key_scene(key)];
+        B = s.B;
+        partial = s.partial != 0;
+        last = level_i == s.level - 1;
+        active = level_i < s.level;
+        if (active)
+        {
+            if (partial)
+                classify_split<G, true>(s, key, level_i, j, refine, inside, first_empty);
+            else
+                classify_split<G, false>(s, key, level_i, j, refine, inside, first_empty);
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1)
+    {
+        refine |= __shfl_xor(refine, off, G);
+        inside |= __shfl_xor(inside, off, G);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            first_empty[c] = min(first_empty[c], uint32_t(__shfl_xor(int(first_empty[c]), off, G)));
+    }
+    uint32_t tests = 0u;
+    if (j == 0 && p < n)
+    {
+        uint32_t dead = 0u;
+        if (active)
+        {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+            {
+                const bool d = first_empty[c] != 0xffffffffu;
+                dead |= d ? (1u << c) : 0u;
+                tests += d ? first_empty[c] + 1u : uint32_t(B);
+            }
+        }
+        uint32_t occ = active ? (refine & ~dead) : 0u;
+        if (partial && last)
+            occ &= inside;
+        child_mask[p] = uint8_t(occ);
+    }
+    uint32_t tsum;
+    block_exclusive_sum(tests, lds, tsum);
+    if (threadIdx.x == 0 && tests_out != nullptr)
+        atomicAdd(tests_out, (unsigned long long)tsum);
+}
+
+__global__ __launch_bounds__(kCarveThreads) void
+carve_count_kernel(const uint8_t* __restrict__ child_mask, uint32_t n, uint32_t* __restrict__ block_counts)
+{
+    __shared__ uint32_t lds[kCarveThreads / 64];
+    const uint32_t t = blockIdx.x * kCarveThreads + threadIdx.x;
+    const uint32_t mask = (t < n) ? child_mask[t] : 0u;
+    uint32_t total;
+    block_exclusive_sum(__popc(mask), lds, total);
+    if (threadIdx.x == 0)
+        block_counts[blockIdx.x] = total;
 }
 
 __global__ __launch_bounds__(kCarveThreads) void
@@ -452,8 +743,33 @@ launch_carve_classify(const uint64_t* parents,
 {
     if (n == 0)
         return;
-    carve_classify_kernel<<<blocks_for(n, kCarveThreads), kCarveThreads, 0, st>>>(parents, n, level_i, scenes,
-                                                                                 child_mask, block_counts, tests);
+    const uint64_t nb = blocks_for(n, kCarveThreads);
+    if (n <= 16384)
+    {
+        carve_classify_split_kernel<64><<<blocks_for(int64_t(n) * 64, 256), 256, 0, st>>>(parents, n, level_i, scenes,
+                                                                                          child_mask, tests);
+        THULL_LAUNCH_CHECK();
+        carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
+    }
+    else if (n <= 65536)
+    {
+        carve_classify_split_kernel<16><<<blocks_for(int64_t(n) * 16, 256), 256, 0, st>>>(parents, n, level_i, scenes,
+                                                                                          child_mask, tests);
+        THULL_LAUNCH_CHECK();
+        carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
+    }
+    else if (n <= 262144)
+    {
+        carve_classify_split_kernel<4><<<blocks_for(int64_t(n) * 4, 256), 256, 0, st>>>(parents, n, level_i, scenes,
+                                                                                        child_mask, tests);
+        THULL_LAUNCH_CHECK();
+        carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
+    }
+    else
+    {
+        carve_classify_kernel<<<nb, kCarveThreads, 0, st>>>(parents, n, level_i, scenes, child_mask, block_counts,
+                                                            tests);
+    }
     THULL_LAUNCH_CHECK();
 }
 
@@ -483,47 +799,127 @@ launch_carve_scatter(const uint64_t* parents,
 // (vertex -> list position) of marching cubes.
 // ===========================================================================
 
+// Leaf cells of the scenes whose target level is `level`: the children that
+// carving step level-1 kept, read from the parents' child masks.  A parent
+// owns a 2x2 block of bits in four cell rows, and both bits of a row sit in
+// one word (2 px is even), so it issues at most four atomics.
 __global__ __launch_bounds__(256) void
-mark_corners_kernel(const uint64_t* __restrict__ cells,
-                    uint32_t n,
-                    int32_t level,
-                    const SceneDev* __restrict__ scenes,
-                    uint32_t* __restrict__ bitmap)
+mark_leaf_cells_kernel(const uint64_t* __restrict__ parents,
+                       uint32_t n,
+                       const uint8_t* __restrict__ child_mask,
+                       int32_t level,
+                       const SceneDev* __restrict__ scenes,
+                       uint32_t* __restrict__ cells)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n)
         return;
-    const uint64_t key = cells[t];
+    const uint32_t m = child_mask[t];
+    if (m == 0u)
+        return;
+    const uint64_t key = parents[t];
     const SceneDev& s = scenes[key_scene(key)];
     if (s.level != level)
         return;
-    const uint32_t x = key_x(key), y = key_y(key), z = key_z(key);
-    const uint64_t R1 = (uint64_t(1) << level) + 1;
-    uint32_t* base = bitmap + s.bitmap_offset;
-    const uint32_t bit = x & 31u;
+    const uint64_t R = uint64_t(1) << level;
+    const uint32_t cx = 2 * key_x(key), cy = 2 * key_y(key), cz = 2 * key_z(key);
+    uint32_t* base = cells + s.cell_offset + (cx >> 5);
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
+        for (int b = 0; b < 2; ++b)
         {
-            uint32_t* w = base + ((uint64_t(z + dz) * R1 + (y + dy)) * s.row_words + (x >> 5));
-            if (bit != 31u)
-                atomicOr(w, 3u << bit);
-            else
-            {
-                atomicOr(w, 1u << 31);
-                atomicOr(w + 1, 1u);
-            }
+            const uint32_t bits = (m >> (2 * b + 4 * c)) & 3u; // children (x=0,1, y=b, z=c)
+            if (bits != 0u)
+                atomicOr(base + (uint64_t(cz + c) * R + (cy + b)) * s.cell_row_words, bits << (cx & 31u));
         }
 }
 
+// Level-0 scenes: the root cell is the only leaf.
+__global__ void
+mark_root_cells_kernel(const SceneDev* __restrict__ scenes, int32_t nscenes, uint32_t* __restrict__ cells)
+{
+    const int32_t s = blockIdx.x * 256 + threadIdx.x;
+    if (s < nscenes && scenes[s].level == 0)
+        cells[scenes[s].cell_offset] = 1u;
+}
+
+// Vertex bitmap = leaf cells dilated by {0,1}^3: vertex (x, y, z) is a corner
+// of cell (x - dx, y - dy, z - dz).  One thread per vertex word, no atomics.
+__global__ __launch_bounds__(256) void
+dilate_cells_kernel(const uint32_t* __restrict__ cells,
+                    uint32_t* __restrict__ verts,
+                    uint64_t nwords,
+                    const uint64_t* __restrict__ scene_word_begin,
+                    const SceneDev* __restrict__ scenes,
+                    int32_t nscenes)
+{
+    const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (w >= nwords)
+        return;
+    int32_t lo = 0, hi = nscenes - 1;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (scene_word_begin[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SceneDev& s = scenes[lo];
+    const uint64_t local = w - scene_word_begin[lo];
+    const uint64_t R = uint64_t(1) << s.level, R1 = R + 1;
+    const uint64_t row = local / s.row_words;
+    const uint32_t xw = uint32_t(local - row * s.row_words);
+    const int64_t y = int64_t(row % R1), z = int64_t(row / R1);
+    const uint32_t rwc = s.cell_row_words;
+    uint32_t out = 0u;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+    {
+        const int64_t cz = z - dz;
+        if (cz < 0 || cz >= int64_t(R))
+            continue;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+        {
+            const int64_t cy = y - dy;
+            if (cy < 0 || cy >= int64_t(R))
+                continue;
+            const uint32_t* rowp = cells + s.cell_offset + (uint64_t(cz) * R + uint64_t(cy)) * rwc;
+            const uint32_t cw = xw < rwc ? rowp[xw] : 0u;
+            const uint32_t pw = (xw > 0 && xw - 1 < rwc) ? rowp[xw - 1] : 0u;
+            out |= cw | (cw << 1) | (pw >> 31);
+        }
+    }
+    verts[w] = out;
+}
+
 void
-launch_mark_corners(const uint64_t* cells, uint32_t n, int32_t level, const SceneDev* scenes, uint32_t* bitmap,
-                    hipStream_t st)
+launch_mark_leaf_cells(const uint64_t* parents, uint32_t n, const uint8_t* child_mask, int32_t level,
+                       const SceneDev* scenes, uint32_t* cells, hipStream_t st)
 {
     if (n == 0)
         return;
-    mark_corners_kernel<<<blocks_for(n, 256), 256, 0, st>>>(cells, n, level, scenes, bitmap);
+    mark_leaf_cells_kernel<<<blocks_for(n, 256), 256, 0, st>>>(parents, n, child_mask, level, scenes, cells);
+    THULL_LAUNCH_CHECK();
+}
+
+void
+launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells, hipStream_t st)
+{
+    mark_root_cells_kernel<<<blocks_for(nscenes, 256), 256, 0, st>>>(scenes, nscenes, cells);
+    THULL_LAUNCH_CHECK();
+}
+
+void
+launch_dilate_cells(const uint32_t* cells, uint32_t* verts, uint64_t nwords, const uint64_t* scene_word_begin,
+                    const SceneDev* scenes, int32_t nscenes, hipStream_t st)
+{
+    if (nwords == 0)
+        return;
+    dilate_cells_kernel<<<blocks_for(int64_t(nwords), 256), 256, 0, st>>>(cells, verts, nwords, scene_word_begin,
+                                                                         scenes, nscenes);
     THULL_LAUNCH_CHECK();
 }
 
@@ -602,15 +998,9 @@ i64_to_f32(int64_t v)
     return float(v);
 }
 
-__global__ __launch_bounds__(256) void
-field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* __restrict__ scenes,
-             float* __restrict__ values)
+__device__ __forceinline__ float
+field_value(const SceneDev& s, uint64_t key)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint64_t key = vertices[i];
-    const SceneDev s = scenes[key_scene(key)];
     const float inv = __builtin_ldexpf(1.0f, -s.level);
     const float wx = grid_to_world(key_x(key), inv, s.length, s.corner[0]);
     const float wy = grid_to_world(key_y(key), inv, s.length, s.corner[1]);
@@ -662,6 +1052,30 @@ field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* 
         const float l0 = lerp_ref(v00, v10, ty);
         const float l1 = lerp_ref(v01, v11, ty);
         value *= lerp_ref(l0, l1, tx);
+    }
+    return value;
+}
+
+__global__ __launch_bounds__(256) void
+field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* __restrict__ scenes,
+             float* __restrict__ values)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t key = vertices[i];
+    const uint32_t sid = key_scene(key);
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+    float value;
+    if (__all(sid == s0))
+    {
+        const SceneDev s = scenes[s0]; // wave-uniform: scalar camera data
+        value = field_value(s, key);
+    }
+    else
+    {
+        const SceneDev s = scenes[sid];
+        value = field_value(s, key);
     }
     values[i] = value;
 }
