@@ -87,7 +87,8 @@ def carve_profile(run_fn) -> dict:
     n = 16
     ms = (ctypes.c_float * n)()
     par, tests, nxt = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
-    _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt))
+    split = (ctypes.c_int32 * n)()
+    _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt, split))
     levels = []
     for i in range(n):
         if par[i] == 0:
@@ -95,19 +96,42 @@ def carve_profile(run_fn) -> dict:
         # algorithmic bytes of carving step i: parents read (4 B), 4 integral reads of 4 B per camera
         # test (early exit), children written (4 B)
         b = 4 * par[i] + 16 * tests[i] + 4 * nxt[i]
-        levels.append(dict(step=i, level=i + 1, ms=float(ms[i]), parents=int(par[i]), camera_tests=int(tests[i]),
-                           children=int(nxt[i]), bytes=int(b)))
+        kernel = "carve_classify_kernel" if split[i] == 1 else f"carve_classify_split_kernel<{split[i]}>"
+        levels.append(dict(step=i, level=i + 1, kernel=kernel, ms=float(ms[i]), parents=int(par[i]),
+                           camera_tests=int(tests[i]), children=int(nxt[i]), bytes=int(b)))
     return dict(levels=levels)
 
 
+def traffic_from_profile() -> dict | None:
+    """HBM bytes per carve_classify_kernel launch of the batched step, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "carve_classify_mixed64_traffic.txt")
+    try:
+        vals = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        return {"fetch": float(vals["fetch_bytes_per_launch"]), "write": float(vals["write_bytes_per_launch"])}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def roofline_from(levels: list[dict]) -> dict:
-    tot_b = sum(lv["bytes"] for lv in levels)
-    tot_ms = sum(lv["ms"] for lv in levels)
-    n = len(levels)
+    """Roofline of carve_classify_kernel (the largest kernel of the batched step) over its launches in
+    one batched step; the camera-split launches of the small levels are other kernels."""
+    sel = [lv for lv in levels if lv["kernel"] == "carve_classify_kernel"]
+    tot_b = sum(lv["bytes"] for lv in sel)
+    tot_ms = sum(lv["ms"] for lv in sel)
+    n = len(sel)
     achieved = tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    tr = traffic_from_profile()
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if tr is None else int(tr["fetch"] + tr["write"]),
             "kernel": "carve_classify_kernel", "launches": n,
+            "levels": [lv["level"] for lv in sel],
             "bytes_per_launch": int(tot_b / max(n, 1)), "avg_launch_ms": round(tot_ms / max(n, 1), 4)}
 
 
@@ -149,6 +173,9 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the level-10 carving measurement of config 2")
+    ap.add_argument("--batched-only", action="store_true", help="skip the per-scene path (for profiling runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one float32 tensor copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -156,6 +183,16 @@ def main() -> None:
         sys.exit(2)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * 2**20 // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
 
     log("rendering mixed workload scenes on the GPU ...")
     scenes = synthetic.mixed_scenes(args.scenes, device=dev)
@@ -165,13 +202,14 @@ def main() -> None:
 
     t_b, meshes_b = timed(lambda: step_batched(scenes), args.steps, args.warmup)
     log(f"batched: {1e3 * t_b / args.steps:.2f} ms/step")
-    t_s, meshes_s = timed(lambda: step_per_scene(scenes), args.steps, args.warmup)
-    log(f"per-scene: {1e3 * t_s / args.steps:.2f} ms/step")
-
-    mismatch = 0
-    for (vb, fb), (vs, fs) in zip(meshes_b, meshes_s):
-        if not (torch.equal(fb, fs) and torch.equal(vb.view(torch.int32), vs.view(torch.int32))):
-            mismatch += 1
+    t_s, mismatch = None, None
+    if not args.batched_only:
+        t_s, meshes_s = timed(lambda: step_per_scene(scenes), args.steps, args.warmup)
+        log(f"per-scene: {1e3 * t_s / args.steps:.2f} ms/step")
+        mismatch = 0
+        for (vb, fb), (vs, fs) in zip(meshes_b, meshes_s):
+            if not (torch.equal(fb, fs) and torch.equal(vb.view(torch.int32), vs.view(torch.int32))):
+                mismatch += 1
 
     prof = carve_profile(lambda: step_batched(scenes))
     roof = roofline_from(prof["levels"])
@@ -192,8 +230,8 @@ def main() -> None:
         "config": {"workload": "mixed64: 64 scenes, 1-8 ellipsoids, 4-48 cameras, 256/512/1024 px, level 6-9, "
                                "masks_partial p=0.3, blur k7 sigma1.5, opengl", "scenes": len(scenes),
                    "mask_gib": round(mask_bytes / 2**30, 3)},
-        "per_scene_hulls_per_s": round(len(scenes) * args.steps / t_s, 3),
-        "batched_over_per_scene": round(t_s / t_b, 3),
+        "per_scene_hulls_per_s": None if t_s is None else round(len(scenes) * args.steps / t_s, 3),
+        "batched_over_per_scene": None if t_s is None else round(t_s / t_b, 3),
         "exact_match": {"batched_vs_per_scene_mismatches": mismatch, "scenes": len(scenes)},
         "roofline": roof,
         "carve_levels": prof["levels"],
@@ -208,7 +246,7 @@ def main() -> None:
         p2 = carve_profile(run2)
         last = p2["levels"][-1]
         result["config2_level10_carve"] = {
-            "level": last["level"], "ms": round(last["ms"], 4), "bytes": last["bytes"],
+            "level": last["level"], "kernel": last["kernel"], "ms": round(last["ms"], 4), "bytes": last["bytes"],
             "camera_tests": last["camera_tests"], "achieved_GBs": round(last["bytes"] / (last["ms"] * 1e-3) / 1e9, 1),
             "frac": round(last["bytes"] / (last["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "camera_tests_per_s": round(last["camera_tests"] / (last["ms"] * 1e-3), 1),

@@ -117,9 +117,10 @@ int thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_
  * (no counterpart in the reference; instrumentation for roofline accounting). */
 int thull_profile_enable(thull_ctx* ctx, int32_t enabled);
 /* Per carving step i of the last run: classification kernel time (ms, 0 when
- * profiling is off), parent cells, camera tests and resulting child cells. */
+ * profiling is off), parent cells, camera tests, resulting child cells and the
+ * kernel used (split = 1: one thread per parent; 4/16/64: lanes per parent). */
 int thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests,
-                        int64_t* next);
+                        int64_t* next, int32_t* split);
 
 /* Marching cubes on a dense volume [1,D,H,W] (value dtype THULL_F32 or THULL_F64)
  * or a coalesced sparse COO volume (indices int64 [4,nnz], values [nnz]).

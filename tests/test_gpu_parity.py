@@ -37,7 +37,8 @@ def _lib():
 
 
 @pytest.mark.parametrize("kind", ["binary", "fractional"])
-@pytest.mark.parametrize("shape", [(3, 17, 33), (2, 256, 300), (1, 1024, 64)])
+@pytest.mark.parametrize("shape", [(3, 17, 33), (2, 256, 300), (1, 1024, 64), (2, 130, 100), (1, 1, 1), (1, 1, 4),
+                                   (1, 65, 4), (2, 64, 36), (1, 3, 2052)])
 def test_integral_image_matches_oracle_and_torch(kind, shape):
     rng = np.random.default_rng(1)
     if kind == "binary":

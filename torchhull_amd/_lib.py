@@ -78,7 +78,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.thull_gaussian_blur.argtypes = [c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_i32, c_float, c_i32,
                                       c_void_p, c_i32, c_void_p]
     L.thull_profile_enable.argtypes = [c_void_p, c_i32]
-    L.thull_profile_carve.argtypes = [c_void_p, c_i32, ctypes.POINTER(c_float), P_i64, P_i64, P_i64]
+    L.thull_profile_carve.argtypes = [c_void_p, c_i32, ctypes.POINTER(c_float), P_i64, P_i64, P_i64,
+                                      ctypes.POINTER(c_i32)]
     L.thull_gaussian_blur_batch.argtypes = [c_void_p, c_i32, ctypes.POINTER(c_void_p), c_i32, P_i64, c_i32, c_float,
                                             c_i32, ctypes.POINTER(c_void_p), c_i32, c_void_p]
     L.thull_integral_image.argtypes = [c_void_p, c_void_p, c_i32, c_i64, c_i64, c_i64, c_void_p, c_void_p]

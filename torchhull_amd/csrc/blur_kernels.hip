@@ -15,6 +15,7 @@
 // float and double, per-operation binary16 rounding for half) and its final
 // division by the weight sum; every other pixel is the converted input.
 
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_fp16.h>
@@ -127,27 +128,22 @@ struct BlurJob
 __device__ __forceinline__ int32_t
 find_blur_job(const BlurJob* jobs, int32_t n, uint64_t blk, bool tiles)
 {
-    int32_t lo = 0, hi = n - 1;
-    while (lo < hi)
-    {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if ((tiles ? jobs[mid].t_begin : jobs[mid].p_begin) <= blk)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return __builtin_amdgcn_readfirstlane(lo);
+    if (tiles)
+        return find_segment_uniform([&](int32_t i) { return jobs[i].t_begin; }, n, blk);
+    return find_segment_uniform([&](int32_t i) { return jobs[i].p_begin; }, n, blk);
 }
 
 constexpr int kBlurBX = 64; // pixel block: 64 columns x RB rows, 256 threads
 
 // Edge pixels (GB:116-142) marked into their tile's flag.  Every writer
 // stores the same value 1, so the plain byte stores need no atomics; flags
-// are zeroed beforehand.
+// are zeroed beforehand.  Wave w of a block tests RB / 4 consecutive rows of
+// the block; all of a thread's pixels are loaded before any flag is stored.
 template <typename Ti, int RB>
 __global__ __launch_bounds__(256) void
 blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t ts)
 {
+    constexpr int R = RB / 4;
     const uint64_t blk = blockIdx.x;
     const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
     const uint32_t local = uint32_t(blk - j.p_begin);
@@ -157,23 +153,85 @@ blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t 
     const int32_t by = int32_t(r / j.bx), bx = int32_t(r - uint32_t(by) * j.bx);
     const int32_t H = j.H, W = j.W;
     const int32_t x = bx * kBlurBX + int32_t(threadIdx.x & 63);
-    if (x >= W)
+    const int32_t yb = by * RB + int32_t(threadIdx.x >> 6) * R;
+    if (x >= W || yb >= H)
         return;
-    const Ti* I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
-    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW;
-    const int32_t tx = x / ts;
-#pragma unroll 4
-    for (int32_t q = 0; q < RB / 4; ++q)
+    const Ti* __restrict__ I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W + x;
+    const bool has_right = x + 1 < W;
+    Ti c[R + 1], rt[R];
+#pragma unroll
+    for (int q = 0; q <= R; ++q)
     {
-        const int32_t y = by * RB + int32_t(threadIdx.x >> 6) + 4 * q;
-        if (y >= H)
+        const bool in = yb + q < H; // row yb + q exists (q == R: the row below the thread's last row)
+        c[q] = in ? I[int64_t(yb + q) * W] : (q > 0 ? c[q - 1] : Ti{});
+        if (q < R)
+            rt[q] = (in && has_right) ? I[int64_t(yb + q) * W + 1] : c[q];
+    }
+    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW + x / ts;
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+    {
+        if (yb + q >= H)
             break;
-        const int64_t o = int64_t(y) * W + x;
-        const Ti c = I[o];
-        const Ti rr = (x + 1 < W) ? I[o + 1] : c;
-        const Ti d = (y + 1 < H) ? I[o + W] : c;
-        if (differs(rr, c) || differs(d, c))
-            F[(y / ts) * j.TW + tx] = 1;
+        if (differs(rt[q], c[q]) || differs(c[q + 1], c[q]))
+            F[((yb + q) / ts) * j.TW] = 1;
+    }
+}
+
+// float32 images with W % 4 == 0: each thread tests 4 consecutive pixels of
+// its rows with 16-byte loads (256 columns per block).
+template <int RB>
+__global__ __launch_bounds__(256) void
+blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t ts)
+{
+    constexpr int R = RB / 4;
+    const uint64_t blk = blockIdx.x;
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const uint32_t local = uint32_t(blk - j.p_begin);
+    const uint32_t per = j.bx * j.by;
+    const int32_t b = int32_t(local / per);
+    const uint32_t r = local - uint32_t(b) * per;
+    const int32_t by = int32_t(r / j.bx), bx = int32_t(r - uint32_t(by) * j.bx);
+    const int32_t H = j.H, W = j.W;
+    const int32_t x = (bx * 64 + int32_t(threadIdx.x & 63)) * 4;
+    const int32_t yb = by * RB + int32_t(threadIdx.x >> 6) * R;
+    if (x >= W || yb >= H)
+        return;
+    const float* __restrict__ I = static_cast<const float*>(j.in) + int64_t(b) * H * W + x;
+    const bool has_right = x + 4 < W;
+    float4 c[R + 1];
+    float rt[R];
+#pragma unroll
+    for (int q = 0; q <= R; ++q)
+    {
+        const bool in = yb + q < H;
+        c[q] = in ? *reinterpret_cast<const float4*>(I + int64_t(yb + q) * W) : (q > 0 ? c[q - 1] : float4{});
+        if (q < R)
+            rt[q] = (in && has_right) ? I[int64_t(yb + q) * W + 4] : c[q].w;
+    }
+    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW;
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+    {
+        if (yb + q >= H)
+            break;
+        const float4 u = c[q], d = c[q + 1];
+        const bool e0 = (u.y != u.x) || (d.x != u.x);
+        const bool e1 = (u.z != u.y) || (d.y != u.y);
+        const bool e2 = (u.w != u.z) || (d.z != u.z);
+        const bool e3 = (rt[q] != u.w) || (d.w != u.w);
+        if (e0 || e1 || e2 || e3)
+        {
+            uint8_t* row = F + ((yb + q) / ts) * j.TW;
+            if (e0)
+                row[x / ts] = 1;
+            if (e1)
+                row[(x + 1) / ts] = 1;
+            if (e2)
+                row[(x + 2) / ts] = 1;
+            if (e3)
+                row[(x + 3) / ts] = 1;
+        }
     }
 }
 
@@ -202,12 +260,13 @@ blur_band_tiles_kernel(const BlurJob* __restrict__ jobs, int32_t njobs)
 // Band pixels of a 64 x RB block are convolved from an LDS tile with a halo
 // of k/2 (reflect-padded at the image border); blocks without band pixels
 // only convert and copy.  `norm` is the reference's sum of the weights in tap
-// order and output precision, the same for every pixel.
-template <typename Ti, typename To, int RB>
+// order and output precision, the same for every pixel.  KT > 0 fixes the
+// kernel size at compile time (k == KT); KT == 0 takes it from `k`.
+template <typename Ti, typename To, int RB, int KT>
 __global__ __launch_bounds__(256) void
 blur_tile_kernel(const BlurJob* __restrict__ jobs,
                  int32_t njobs,
-                 int32_t k,
+                 int32_t k_rt,
                  int32_t ts,
                  bool sparse,
                  const To* __restrict__ weights,
@@ -216,6 +275,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     extern __shared__ unsigned char smem_raw[];
     To* tile = reinterpret_cast<To*>(smem_raw);
     constexpr int P = RB / 4; // pixels per thread
+    const int32_t k = KT > 0 ? KT : k_rt;
     const uint64_t blk = blockIdx.x;
     const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
     const uint32_t local = uint32_t(blk - j.p_begin);
@@ -227,53 +287,70 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     const int32_t h = k / 2;
     const int32_t LW = kBlurBX + 2 * h, LH = RB + 2 * h;
     const int32_t x0 = bx * kBlurBX, y0 = by * RB;
-    const Ti* I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
-    To* O = static_cast<To*>(j.out) + int64_t(b) * H * W;
+    const Ti* __restrict__ I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
+    To* __restrict__ O = static_cast<To*>(j.out) + int64_t(b) * H * W;
     const int32_t lx = int32_t(threadIdx.x & 63), ly0 = int32_t(threadIdx.x >> 6);
     const int32_t x = x0 + lx;
     const bool dense = !sparse && k > 1;
     const bool use_flags = sparse && k > 1;
-    const uint8_t* band_tiles = use_flags ? j.flags + uint64_t(j.B + b) * j.TH * j.TW : nullptr;
-    uint32_t band = 0u; // bit q: pixel (x, y0 + ly0 + 4q)
-    if (x < W)
+    // pixel q of this thread: (x, y0 + ly0 + 4q); rows past H count as absent
+    int32_t nq = (H - (y0 + ly0) + 3) / 4;
+    nq = x < W ? min(max(nq, 0), P) : 0;
+    uint32_t band = 0u;
+    if (dense)
+        band = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1u);
+    else if (use_flags)
     {
-        for (int32_t q = 0; q < P; ++q)
-        {
-            const int32_t y = y0 + ly0 + 4 * q;
-            if (y >= H)
-                break;
-            const bool bq = dense || (use_flags && band_tiles[(y / ts) * j.TW + x / ts] != 0);
-            band |= bq ? (1u << q) : 0u;
-        }
+        const uint8_t* __restrict__ bt = j.flags + uint64_t(j.B + b) * j.TH * j.TW + x / ts;
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if (q < nq && bt[((y0 + ly0 + 4 * q) / ts) * j.TW] != 0)
+                band |= 1u << q;
     }
     if (!__syncthreads_or(band != 0u))
     {
-        if (x < W)
-            for (int32_t q = 0; q < P; ++q)
-            {
-                const int32_t y = y0 + ly0 + 4 * q;
-                if (y >= H)
-                    break;
-                O[int64_t(y) * W + x] = convert_in<To>(I[int64_t(y) * W + x]); // images.clone().to(dtype) (GB:444)
-            }
+        To v[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if (q < nq)
+                v[q] = convert_in<To>(I[int64_t(y0 + ly0 + 4 * q) * W + x]); // images.clone().to(dtype) (GB:444)
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if (q < nq)
+                O[int64_t(y0 + ly0 + 4 * q) * W + x] = v[q];
         return;
     }
-    // stage the block and its halo, converted to the output type (GB:221)
-    for (int32_t q = threadIdx.x; q < LW * LH; q += 256)
+    // stage the block and its halo, converted to the output type (GB:221): wave w stages rows
+    // w, w + 4, ... in groups of 4 rows whose loads are issued together
+    for (int32_t cx = lx; cx < LW; cx += 64)
     {
-        const int32_t ty = q / LW, tx = q - ty * LW;
-        const int64_t yy = reflect_index(y0 + ty - h, H), xx = reflect_index(x0 + tx - h, W);
-        tile[q] = convert_in<To>(I[yy * W + xx]);
+        const int64_t xx = reflect_index(x0 + cx - h, W);
+        for (int32_t ty0 = ly0; ty0 < LH; ty0 += 16)
+        {
+            To v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                const int32_t ty = ty0 + 4 * u;
+                if (ty < LH)
+                    v[u] = convert_in<To>(I[reflect_index(y0 + ty - h, H) * W + xx]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+            {
+                const int32_t ty = ty0 + 4 * u;
+                if (ty < LH)
+                    tile[ty * LW + cx] = v[u];
+            }
+        }
     }
     __syncthreads();
     if (x >= W)
         return;
-    for (int32_t q = 0; q < P; ++q)
+    for (int32_t q = 0; q < nq; ++q)
     {
         const int32_t ly = ly0 + 4 * q;
         const int32_t y = y0 + ly;
-        if (y >= H)
-            break;
         To result;
         if (!(band & (1u << q)))
             result = tile[(ly + h) * LW + lx + h];
@@ -281,13 +358,25 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
         {
             // GB:217-230: x offset i outer, y offset j inner
             To acc = OutOps<To>::zero();
-            for (int32_t i = -h; i <= h; ++i)
+#pragma unroll
+            for (int32_t i = 0; i < (KT > 0 ? KT : 1); ++i)
             {
-                const To* col = tile + (lx + h + i);
-                const To* wcol = weights + (i + h) * k;
-                for (int32_t jj = -h; jj <= h; ++jj)
-                    acc = OutOps<To>::mac(acc, wcol[jj + h], col[(ly + h + jj) * LW]);
+                if (KT > 0)
+                {
+                    const To* col = tile + (lx + i) + ly * LW;
+#pragma unroll
+                    for (int32_t jj = 0; jj < KT; ++jj)
+                        acc = OutOps<To>::mac(acc, weights[i * KT + jj], col[jj * LW]);
+                }
             }
+            if (KT == 0)
+                for (int32_t i = 0; i < k; ++i)
+                {
+                    const To* col = tile + (lx + i) + ly * LW;
+                    const To* wcol = weights + i * k;
+                    for (int32_t jj = 0; jj < k; ++jj)
+                        acc = OutOps<To>::mac(acc, wcol[jj], col[jj * LW]);
+                }
             result = sparse ? OutOps<To>::div(acc, norm) : acc;
         }
         O[int64_t(y) * W + x] = result;
@@ -311,17 +400,22 @@ blur_job_bytes(size_t n)
 
 template <typename Ti, typename To, int RB>
 static void
-run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, hipStream_t st)
+run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm,
+            uint8_t* flags, hipStream_t st)
 {
     const int32_t ts = k / 2;
     const bool use_band = sparse && k > 1;
-    uint64_t pb = 0, tbk = 0;
+    // the edge pass reads 4 pixels per thread with 16-byte loads for float32 images of width % 4 == 0
+    bool vec = std::is_same_v<Ti, float>;
+    for (auto& j : jobs)
+        vec = vec && (j.W % 4 == 0) && (reinterpret_cast<uintptr_t>(j.in) % 16 == 0);
+    uint64_t pb = 0, tbk = 0, eb = 0, fbytes = 0;
     for (auto& j : jobs)
     {
-        j.bx = blocks_for(j.W, kBlurBX);
         j.by = blocks_for(j.H, RB);
-        j.p_begin = pb;
-        pb += uint64_t(j.bx) * j.by * j.B;
+        j.bx = blocks_for(j.W, vec ? 4 * kBlurBX : kBlurBX);
+        j.p_begin = eb; // edge-pass blocks first
+        eb += uint64_t(j.bx) * j.by * j.B;
         if (use_band)
         {
             j.TH = j.H / ts + 1;
@@ -329,24 +423,58 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
             j.tb = uint32_t(blocks_for(int64_t(j.TH) * j.TW * j.B, 256));
             j.t_begin = tbk;
             tbk += j.tb;
-            THULL_HIP(hipMemsetAsync(j.flags, 0, size_t(j.B) * j.TH * j.TW, st));
+            j.flags = flags + fbytes;
+            fbytes += 2 * uint64_t(j.B) * j.TH * j.TW;
         }
+    }
+    if (use_band && eb > 0)
+    {
+        THULL_CHECK(eb < (uint64_t(1) << 31) && tbk < (uint64_t(1) << 31), "blur batch too large");
+        THULL_HIP(hipMemsetAsync(flags, 0, fbytes, st));
+        THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
+        if (vec)
+        {
+            if constexpr (std::is_same_v<Ti, float>)
+                blur_edge_flags_f32x4_kernel<RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
+        }
+        else
+            blur_edge_flags_kernel<Ti, RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
+        THULL_LAUNCH_CHECK();
+        blur_band_tiles_kernel<<<unsigned(tbk), 256, 0, st>>>(djobs, int32_t(jobs.size()));
+        THULL_LAUNCH_CHECK();
+        // the tile pass uses its own block layout; the host table must stay valid until the copy ran
+        THULL_HIP(hipStreamSynchronize(st));
+    }
+    for (auto& j : jobs)
+    {
+        j.bx = blocks_for(j.W, kBlurBX);
+        j.p_begin = pb;
+        pb += uint64_t(j.bx) * j.by * j.B;
     }
     if (pb == 0)
         return;
-    THULL_CHECK(pb < (uint64_t(1) << 31) && tbk < (uint64_t(1) << 31), "blur batch too large");
+    THULL_CHECK(pb < (uint64_t(1) << 31), "blur batch too large");
     THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
     const int32_t n = int32_t(jobs.size());
-    if (use_band)
-    {
-        blur_edge_flags_kernel<Ti, RB><<<unsigned(pb), 256, 0, st>>>(djobs, n, ts);
-        THULL_LAUNCH_CHECK();
-        blur_band_tiles_kernel<<<unsigned(tbk), 256, 0, st>>>(djobs, n);
-        THULL_LAUNCH_CHECK();
-    }
     const int32_t h = k / 2;
     const size_t lds = size_t(kBlurBX + 2 * h) * (RB + 2 * h) * sizeof(To);
-    blur_tile_kernel<Ti, To, RB><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, ts > 0 ? ts : 1, sparse, weights, norm);
+    const int32_t tsl = ts > 0 ? ts : 1;
+    // float outputs of the common odd sizes take a kernel with the taps unrolled
+    constexpr bool spec = std::is_same_v<To, float> && (std::is_same_v<Ti, float> || std::is_same_v<Ti, uint8_t>);
+    if (spec && k == 3)
+        blur_tile_kernel<Ti, To, RB, spec ? 3 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
+                                                                                   norm);
+    else if (spec && k == 5)
+        blur_tile_kernel<Ti, To, RB, spec ? 5 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
+                                                                                   norm);
+    else if (spec && k == 7)
+        blur_tile_kernel<Ti, To, RB, spec ? 7 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
+                                                                                   norm);
+    else if (spec && k == 9)
+        blur_tile_kernel<Ti, To, RB, spec ? 9 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
+                                                                                   norm);
+    else
+        blur_tile_kernel<Ti, To, RB, 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights, norm);
     THULL_LAUNCH_CHECK();
     // the job table is pageable host memory read by an async copy
     THULL_HIP(hipStreamSynchronize(st));
@@ -354,13 +482,14 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
 
 template <typename Ti, typename To>
 static void
-run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, hipStream_t st)
+run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, uint8_t* flags,
+         hipStream_t st)
 {
     const int32_t h = k / 2;
     if (size_t(kBlurBX + 2 * h) * (64 + 2 * h) * sizeof(To) <= 48 * 1024)
-        run_blur_rb<Ti, To, 64>(jobs, djobs, k, sparse, weights, norm, st);
+        run_blur_rb<Ti, To, 64>(jobs, djobs, k, sparse, weights, norm, flags, st);
     else if (size_t(kBlurBX + 2 * h) * (16 + 2 * h) * sizeof(To) <= 48 * 1024)
-        run_blur_rb<Ti, To, 16>(jobs, djobs, k, sparse, weights, norm, st);
+        run_blur_rb<Ti, To, 16>(jobs, djobs, k, sparse, weights, norm, flags, st);
     else
         throw Error("kernel_size too large for the LDS tile");
 }
@@ -368,18 +497,19 @@ run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const T
 template <typename Ti>
 static void
 run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const void* weights, double norm,
-             int32_t out_dtype, hipStream_t st)
+             int32_t out_dtype, uint8_t* flags, hipStream_t st)
 {
     switch (out_dtype)
     {
     case kF16:
-        run_blur<Ti, __half>(jobs, djobs, k, sparse, static_cast<const __half*>(weights), __float2half(float(norm)), st);
+        run_blur<Ti, __half>(jobs, djobs, k, sparse, static_cast<const __half*>(weights), __float2half(float(norm)),
+                             flags, st);
         break;
     case kF32:
-        run_blur<Ti, float>(jobs, djobs, k, sparse, static_cast<const float*>(weights), float(norm), st);
+        run_blur<Ti, float>(jobs, djobs, k, sparse, static_cast<const float*>(weights), float(norm), flags, st);
         break;
     case kF64:
-        run_blur<Ti, double>(jobs, djobs, k, sparse, static_cast<const double*>(weights), norm, st);
+        run_blur<Ti, double>(jobs, djobs, k, sparse, static_cast<const double*>(weights), norm, flags, st);
         break;
     default:
         throw Error("gaussian_blur: output dtype must be floating point");
@@ -391,7 +521,8 @@ run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, con
 // weight sum in the output type (exactly representable in double).
 void
 launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int k, bool sparse,
-                           const void* weights, double norm, int32_t out_dtype, void* job_buf, hipStream_t st)
+                           const void* weights, double norm, int32_t out_dtype, uint8_t* flags, void* job_buf,
+                           hipStream_t st)
 {
     std::vector<BlurJob> jobs;
     for (int32_t i = 0; i < n; ++i)
@@ -404,7 +535,6 @@ launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, i
         BlurJob j{};
         j.in = im.in;
         j.out = im.out;
-        j.flags = im.flags;
         j.B = int32_t(im.B);
         j.H = int32_t(im.H);
         j.W = int32_t(im.W);
@@ -417,28 +547,28 @@ launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, i
     {
     case kBool:
     case kU8:
-        run_blur_out<uint8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<uint8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI8:
-        run_blur_out<int8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<int8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI16:
-        run_blur_out<int16_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<int16_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI32:
-        run_blur_out<int32_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<int32_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI64:
-        run_blur_out<int64_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<int64_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF16:
-        run_blur_out<__half>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<__half>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF32:
-        run_blur_out<float>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<float>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF64:
-        run_blur_out<double>(jobs, djobs, k, sparse, weights, norm, out_dtype, st);
+        run_blur_out<double>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     default:
         throw Error("gaussian_blur: unsupported input dtype");

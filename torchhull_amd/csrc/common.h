@@ -176,6 +176,62 @@ struct SceneDev
 };
 
 // ---------------------------------------------------------------------------
+// Segment lookup: index of the last segment whose first element is <= v, for
+// a non-decreasing table begin(0..n-1).
+// ---------------------------------------------------------------------------
+
+// v wave-uniform, every lane of the wave active.  Up to 64 segments take one
+// load per lane and a ballot; larger tables use a binary search.
+template <typename F>
+__device__ __forceinline__ int32_t
+find_segment_uniform(F begin, int32_t n, uint64_t v)
+{
+    if (n <= 64)
+    {
+        const int32_t lane = int32_t(threadIdx.x & 63);
+        const bool le = lane < n && begin(lane) <= v;
+        const uint64_t m = __ballot(le);
+        return m == 0 ? 0 : 63 - __builtin_clzll(m);
+    }
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (begin(mid) <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// v non-decreasing with the lane index (lane 0 holds the wave's smallest v),
+// every lane of the wave active: the segment of lane 0's v, then a short walk
+// forward per lane.
+template <typename F>
+__device__ __forceinline__ int32_t
+find_segment_rising(F begin, int32_t n, uint64_t v)
+{
+    const uint64_t v0 = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(v >> 32))) << 32) |
+                        uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(v)));
+    int32_t s = find_segment_uniform(begin, n, v0);
+    while (s + 1 < n && begin(s + 1) <= v)
+        ++s;
+    return s;
+}
+
+// Quotient of a 64-bit value by a 32-bit divisor, through the 32-bit divide
+// when the dividend fits (the common case; 64-bit division is a long
+// software sequence).
+__device__ __forceinline__ uint64_t
+div_u64_u32(uint64_t a, uint32_t b)
+{
+    if ((a >> 32) == 0u)
+        return uint32_t(a) / b;
+    return a / b;
+}
+
+// ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 

@@ -157,6 +157,7 @@ struct Context
     std::vector<hipEvent_t> ev_pool;
     std::vector<float> carve_ms;
     std::vector<int64_t> carve_parents, carve_next;
+    std::vector<int32_t> carve_split;
 
     hipEvent_t
     event(size_t i)
@@ -332,6 +333,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     ctx.carve_ms.clear();
     ctx.carve_parents.clear();
     ctx.carve_next.clear();
+    ctx.carve_split.clear();
     ctx.vertices = nullptr;
     ctx.values = nullptr;
     ctx.verts = nullptr;
@@ -422,7 +424,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         THULL_HIP(hipMemsetAsync(dev_tests, 0, sizeof(unsigned long long), st));
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i), st));
-        launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests, st);
+        const int32_t split = launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests, st);
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i + 1), st));
         exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st);
@@ -434,6 +436,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         ctx.level_children.push_back(int64_t(count) * 8);
         ctx.carve_parents.push_back(int64_t(count));
         ctx.carve_next.push_back(int64_t(next));
+        ctx.carve_split.push_back(split);
         if (ctx.profile)
         {
             float ms = 0.0f;
@@ -792,7 +795,8 @@ thull_profile_enable(thull_ctx* ctx, int32_t enabled)
 }
 
 int
-thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests, int64_t* next)
+thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests, int64_t* next,
+                    int32_t* split)
 {
     return guarded([&] {
         THULL_CHECK(ctx != nullptr, "ctx is null");
@@ -808,6 +812,8 @@ thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* pare
                 tests[i] = have ? c.level_tests[i] : 0;
             if (next)
                 next[i] = have ? c.carve_next[i] : 0;
+            if (split)
+                split[i] = have ? c.carve_split[i] : 0;
         }
     });
 }
@@ -1136,10 +1142,11 @@ thull_gaussian_blur(thull_ctx* ctx,
         }
         if (B == 0 || H == 0 || W == 0)
             return;
-        BlurImage im{ images, out, nullptr, B, H, W };
+        BlurImage im{ images, out, B, H, W };
+        uint8_t* flags = nullptr;
         if (is_sparse && k > 1)
-            im.flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
-        launch_gaussian_blur_batch(&im, 1, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype,
+            flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
+        launch_gaussian_blur_batch(&im, 1, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype, flags,
                                    c.pool.alloc_bytes(blur_job_bytes(1)), st);
     });
 }
@@ -1177,16 +1184,18 @@ thull_gaussian_blur_batch(thull_ctx* ctx,
         if (it == c.blur_weights.end())
             it = c.blur_weights.emplace(key, make_blur_weights(k, sigma, out_dtype, is_sparse)).first;
         std::vector<BlurImage> ims(n);
+        uint64_t fbytes = 0;
         for (int32_t i = 0; i < n; ++i)
         {
             const int64_t B = sizes[3 * i], H = sizes[3 * i + 1], W = sizes[3 * i + 2];
             THULL_CHECK(B >= 0 && H >= 0 && W >= 0, "sizes must be non-negative");
-            ims[i] = BlurImage{ images[i], outs[i], nullptr, B, H, W };
+            ims[i] = BlurImage{ images[i], outs[i], B, H, W };
             if (is_sparse && k > 1)
-                ims[i].flags = c.pool.alloc<uint8_t>(std::max<uint64_t>(1, blur_tile_flag_bytes(B, H, W, k)));
+                fbytes += uint64_t(blur_tile_flag_bytes(B, H, W, k));
         }
+        uint8_t* flags = fbytes ? c.pool.alloc<uint8_t>(fbytes) : nullptr;
         launch_gaussian_blur_batch(ims.data(), n, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype,
-                                   c.pool.alloc_bytes(blur_job_bytes(std::max(n, 1))), st);
+                                   flags, c.pool.alloc_bytes(blur_job_bytes(std::max(n, 1))), st);
     });
 }
 

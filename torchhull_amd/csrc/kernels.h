@@ -32,8 +32,11 @@ constexpr int kCarveThreads = 256;
 // Classifies the 8 children of each parent; writes an 8-bit occupancy mask per
 // parent and the number of occupied children per block of kCarveThreads
 // parents.  `tests` accumulates the number of camera tests (early exit).
-void launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level_i, const SceneDev* scenes,
-                           uint8_t* child_mask, uint32_t* block_counts, unsigned long long* tests, hipStream_t st);
+// Returns the number of lanes that share one parent's cameras (1: one thread
+// per parent, carve_classify_kernel; 4/16/64: carve_classify_split_kernel<G>).
+int32_t launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level_i, const SceneDev* scenes,
+                              uint8_t* child_mask, uint32_t* block_counts, unsigned long long* tests, hipStream_t st);
+int32_t carve_split_factor(uint32_t n);
 
 // Writes the occupied children of every parent, parent-major and child-minor,
 // at `block_offsets[block]` + the in-block prefix.
@@ -121,12 +124,13 @@ struct BlurImage
 {
     const void* in;  // [B,H,W]
     void* out;       // [B,H,W] of the output dtype
-    uint8_t* flags;  // blur_tile_flag_bytes(B, H, W, k) bytes (sparse, k > 1)
     int64_t B, H, W;
 };
 int64_t blur_tile_flag_bytes(int64_t B, int64_t H, int64_t W, int kernel_size);
 size_t blur_job_bytes(size_t n);
+// `flags`: sum over images of blur_tile_flag_bytes(B, H, W, k) bytes (sparse, k > 1).
 void launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int kernel_size, bool sparse,
-                                const void* weights, double norm, int32_t out_dtype, void* job_buf, hipStream_t st);
+                                const void* weights, double norm, int32_t out_dtype, uint8_t* flags, void* job_buf,
+                                hipStream_t st);
 
 } // namespace thull

@@ -75,8 +75,9 @@ vertex_ref(const McPlan& p, uint64_t i)
     if (p.dense)
     {
         const uint64_t sx = p.dense_dims[0], sy = p.dense_dims[1];
-        const uint64_t row = i / sx;
-        return VertexRef{ 0u, uint32_t(i - row * sx), uint32_t(row % sy), uint32_t(row / sy) };
+        const uint64_t row = div_u64_u32(i, uint32_t(sx));
+        const uint64_t z = div_u64_u32(row, uint32_t(sy));
+        return VertexRef{ 0u, uint32_t(i - row * sx), uint32_t(row - z * sy), uint32_t(z) };
     }
     const uint64_t k = p.vertices[i];
     return VertexRef{ key_scene(k), key_x(k), key_y(k), key_z(k) };
@@ -482,9 +483,10 @@ extract_grid_vertices_kernel(const uint32_t* __restrict__ bitmap, const uint32_t
     if (word == 0u)
         return;
     const McGrid g = *grid;
-    const uint64_t row = w / g.row_words;
+    const uint64_t row = div_u64_u32(w, g.row_words);
     const uint32_t xw = uint32_t(w - row * g.row_words);
-    const uint32_t y = uint32_t(row % g.dims[1]), z = uint32_t(row / g.dims[1]);
+    const uint64_t zq = div_u64_u32(row, g.dims[1]);
+    const uint32_t y = uint32_t(row - zq * g.dims[1]), z = uint32_t(zq);
     uint32_t pos = prefix[w];
     while (word != 0u)
     {

@@ -68,6 +68,7 @@ struct IntegralJob
     int32_t ld;
     int32_t dtype;
     int32_t B, H, W;
+    int32_t vec4;         // float32 masks, W % 4 == 0, 16-byte aligned: 4 columns per thread
     uint32_t col_blocks;  // blocks per camera of the column pass
     uint32_t row_blocks;  // blocks per camera of the row pass
     uint64_t col_begin;   // first block of this job in the column pass
@@ -77,16 +78,9 @@ struct IntegralJob
 __device__ __forceinline__ int32_t
 find_job(const IntegralJob* jobs, int32_t n, uint64_t blk, bool rows)
 {
-    int32_t lo = 0, hi = n - 1;
-    while (lo < hi)
-    {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if ((rows ? jobs[mid].row_begin : jobs[mid].col_begin) <= blk)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
+    if (rows)
+        return find_segment_uniform([&](int32_t i) { return jobs[i].row_begin; }, n, blk);
+    return find_segment_uniform([&](int32_t i) { return jobs[i].col_begin; }, n, blk);
 }
 
 template <typename T>
@@ -116,7 +110,51 @@ integral_cols(const T* __restrict__ src, float* __restrict__ dst, int32_t H, int
     }
 }
 
-// Column pass: one thread per column (coalesced across the wave), sequential over rows.
+// Column pass of 4 adjacent float32 columns per thread: 16-byte loads, four
+// independent chains.
+__device__ __forceinline__ void
+integral_cols4(const float* __restrict__ src, float* __restrict__ dst, int32_t H, int32_t W, int32_t ld)
+{
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    int32_t y = 0;
+    constexpr int U = 8;
+    for (; y + U <= H; y += U)
+    {
+        float4 v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            v[k] = *reinterpret_cast<const float4*>(src + int64_t(y + k) * W);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+        {
+            a0 = a0 + v[k].x;
+            a1 = a1 + v[k].y;
+            a2 = a2 + v[k].z;
+            a3 = a3 + v[k].w;
+            float* d = dst + int64_t(y + k) * ld;
+            d[0] = a0;
+            d[1] = a1;
+            d[2] = a2;
+            d[3] = a3;
+        }
+    }
+    for (; y < H; ++y)
+    {
+        const float4 v = *reinterpret_cast<const float4*>(src + int64_t(y) * W);
+        a0 = a0 + v.x;
+        a1 = a1 + v.y;
+        a2 = a2 + v.z;
+        a3 = a3 + v.w;
+        float* d = dst + int64_t(y) * ld;
+        d[0] = a0;
+        d[1] = a1;
+        d[2] = a2;
+        d[3] = a3;
+    }
+}
+
+// Column pass: one thread per column (or per 4 columns), coalesced across the
+// wave, sequential over rows.
 __global__ __launch_bounds__(256) void
 integral_cols_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
 {
@@ -124,7 +162,17 @@ integral_cols_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     const IntegralJob j = jobs[find_job(jobs, njobs, blk, false)];
     const uint64_t local = blk - j.col_begin;
     const int32_t b = int32_t(local / j.col_blocks);
-    const int32_t x = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    const int32_t t = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    if (j.vec4)
+    {
+        const int32_t x = 4 * t;
+        if (x >= j.W)
+            return;
+        integral_cols4(static_cast<const float*>(j.masks) + int64_t(b) * j.H * j.W + x, j.out + b * j.plane + x, j.H,
+                       j.W, j.ld);
+        return;
+    }
+    const int32_t x = t;
     if (x >= j.W)
         return;
     const int64_t in_off = int64_t(b) * j.H * j.W + x;
@@ -135,40 +183,74 @@ integral_cols_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
         integral_cols(static_cast<const uint8_t*>(j.masks) + in_off, dst, j.H, j.W, j.ld);
 }
 
-// Row pass in place: one wavefront owns 64 rows and walks 64x64 tiles through
-// LDS, so loads and stores stay coalesced while every lane runs the
-// sequential chain of its own row.
+// Row pass in place: one wavefront owns 64 rows and walks 64 x 32 tiles
+// through LDS, so loads and stores stay coalesced (two 128-byte row segments
+// per instruction) while every lane runs the sequential chain of its own row.
+// The next tile is loaded into registers while the current one is scanned.
+constexpr int kRowTileW = 32;
+
 __global__ __launch_bounds__(64) void
 integral_rows_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
 {
-    __shared__ float tile[64][65];
+    __shared__ float tile[64][kRowTileW + 1];
     const uint64_t blk = blockIdx.x;
     const IntegralJob j = jobs[find_job(jobs, njobs, blk, true)];
     const uint64_t local = blk - j.row_begin;
     const int32_t b = int32_t(local / j.row_blocks);
     const int32_t r0 = int32_t(local - uint64_t(b) * j.row_blocks) * 64;
     const int lane = threadIdx.x;
-    float* img = j.out + b * j.plane;
     const int32_t rows = min(64, j.H - r0);
     const int32_t W = j.W, ld = j.ld;
+    // element e of a lane's tile share: row 2e + lane / 32, column lane % 32
+    const int32_t lr = lane >> 5, lc = lane & 31;
+    float* base = j.out + b * j.plane + int64_t(r0 + lr) * ld + lc;
+    const int32_t nl = (rows - lr + 1) >> 1; // elements of this lane inside the strip
+    float pre[32];
+#pragma unroll
+    for (int e = 0; e < 32; ++e)
+        pre[e] = (e < nl && lc < W) ? base[int64_t(2 * e) * ld] : 0.0f;
     float acc = 0.0f;
-    for (int32_t c0 = 0; c0 < W; c0 += 64)
+    for (int32_t c0 = 0; c0 < W; c0 += kRowTileW)
     {
-        const int32_t cols = min(64, W - c0);
-        if (lane < cols)
-            for (int r = 0; r < rows; ++r)
-                tile[r][lane] = img[int64_t(r0 + r) * ld + c0 + lane];
+        const int32_t cols = min(kRowTileW, W - c0);
+#pragma unroll
+        for (int e = 0; e < 32; ++e)
+            tile[2 * e + lr][lc] = pre[e];
         __syncthreads();
+        const int32_t c1 = c0 + kRowTileW;
+        if (c1 < W)
+        {
+            const bool cin = c1 + lc < W;
+#pragma unroll
+            for (int e = 0; e < 32; ++e)
+                pre[e] = (e < nl && cin) ? base[int64_t(2 * e) * ld + c1] : 0.0f;
+        }
         if (lane < rows)
-            for (int c = 0; c < cols; ++c)
+        {
+            if (cols == kRowTileW)
             {
-                acc = acc + tile[lane][c];
-                tile[lane][c] = acc;
+#pragma unroll
+                for (int c = 0; c < kRowTileW; ++c)
+                {
+                    acc = acc + tile[lane][c];
+                    tile[lane][c] = acc;
+                }
             }
+            else
+                for (int c = 0; c < cols; ++c)
+                {
+                    acc = acc + tile[lane][c];
+                    tile[lane][c] = acc;
+                }
+        }
         __syncthreads();
-        if (lane < cols)
-            for (int r = 0; r < rows; ++r)
-                img[int64_t(r0 + r) * ld + c0 + lane] = tile[r][lane];
+        if (lc < cols)
+        {
+#pragma unroll
+            for (int e = 0; e < 32; ++e)
+                if (e < nl)
+                    base[int64_t(2 * e) * ld + c0] = tile[2 * e + lr][lc];
+        }
         __syncthreads();
     }
 }
@@ -199,7 +281,8 @@ integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, bool bord
     uint64_t cols = 0, rows = 0;
     for (auto& j : jobs)
     {
-        j.col_blocks = blocks_for(j.W, 256);
+        j.vec4 = j.dtype == kF32 && j.W % 4 == 0 && reinterpret_cast<uintptr_t>(j.masks) % 16 == 0;
+        j.col_blocks = blocks_for(j.vec4 ? j.W / 4 : j.W, 256);
         j.row_blocks = blocks_for(j.H, 64);
         j.col_begin = cols;
         j.row_begin = rows;
@@ -238,7 +321,7 @@ launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, in
         return;
     THULL_CHECK(dtype == kF32 || dtype == kU8 || dtype == kBool, "integral image: unsupported mask element type");
     std::vector<IntegralJob> jobs(1);
-    jobs[0] = IntegralJob{ masks, out, H * W, int32_t(W), dtype, int32_t(B), int32_t(H), int32_t(W), 0, 0, 0, 0 };
+    jobs[0] = IntegralJob{ masks, out, H * W, int32_t(W), dtype, int32_t(B), int32_t(H), int32_t(W), 0, 0, 0, 0, 0 };
     integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), false, st);
 }
 
@@ -254,7 +337,7 @@ launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_b
         THULL_CHECK(r.dtype == kF32 || r.dtype == kU8 || r.dtype == kBool,
                     "integral image: unsupported mask element type");
         jobs.push_back(IntegralJob{ r.masks, r.padded + (r.W + 1) + 1, (r.H + 1) * (r.W + 1), int32_t(r.W + 1),
-                                    r.dtype, int32_t(r.B), int32_t(r.H), int32_t(r.W), 0, 0, 0, 0 });
+                                    r.dtype, int32_t(r.B), int32_t(r.H), int32_t(r.W), 0, 0, 0, 0, 0 });
     }
     integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), true, st);
 }
@@ -731,7 +814,7 @@ carve_scatter_kernel(const uint64_t* __restrict__ parents,
             children[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
 }
 
-void
+int32_t
 launch_carve_classify(const uint64_t* parents,
                       uint32_t n,
                       int32_t level_i,
@@ -742,23 +825,24 @@ launch_carve_classify(const uint64_t* parents,
                       hipStream_t st)
 {
     if (n == 0)
-        return;
+        return 0;
+    const int32_t split = carve_split_factor(n);
     const uint64_t nb = blocks_for(n, kCarveThreads);
-    if (n <= 16384)
+    if (split == 64)
     {
         carve_classify_split_kernel<64><<<blocks_for(int64_t(n) * 64, 256), 256, 0, st>>>(parents, n, level_i, scenes,
                                                                                           child_mask, tests);
         THULL_LAUNCH_CHECK();
         carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
     }
-    else if (n <= 65536)
+    else if (split == 16)
     {
         carve_classify_split_kernel<16><<<blocks_for(int64_t(n) * 16, 256), 256, 0, st>>>(parents, n, level_i, scenes,
                                                                                           child_mask, tests);
         THULL_LAUNCH_CHECK();
         carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
     }
-    else if (n <= 262144)
+    else if (split == 4)
     {
         carve_classify_split_kernel<4><<<blocks_for(int64_t(n) * 4, 256), 256, 0, st>>>(parents, n, level_i, scenes,
                                                                                         child_mask, tests);
@@ -771,6 +855,20 @@ launch_carve_classify(const uint64_t* parents,
                                                             tests);
     }
     THULL_LAUNCH_CHECK();
+    return split;
+}
+
+int32_t
+carve_split_factor(uint32_t n)
+{
+    // small levels split each parent's cameras over G lanes to fill the chip
+    if (n <= 16384)
+        return 64;
+    if (n <= 65536)
+        return 16;
+    if (n <= 262144)
+        return 4;
+    return 1;
 }
 
 void
@@ -855,23 +953,17 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
                     int32_t nscenes)
 {
     const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    // all lanes take part in the lookup ballot before any exits
+    const int32_t lo = find_segment_rising([&](int32_t i) { return scene_word_begin[i]; }, nscenes, w);
     if (w >= nwords)
         return;
-    int32_t lo = 0, hi = nscenes - 1;
-    while (lo < hi)
-    {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (scene_word_begin[mid] <= w)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
     const SceneDev& s = scenes[lo];
     const uint64_t local = w - scene_word_begin[lo];
     const uint64_t R = uint64_t(1) << s.level, R1 = R + 1;
-    const uint64_t row = local / s.row_words;
+    const uint64_t row = div_u64_u32(local, s.row_words);
     const uint32_t xw = uint32_t(local - row * s.row_words);
-    const int64_t y = int64_t(row % R1), z = int64_t(row / R1);
+    const uint64_t zq = div_u64_u32(row, uint32_t(R1));
+    const int64_t y = int64_t(row - zq * R1), z = int64_t(zq);
     const uint32_t rwc = s.cell_row_words;
     uint32_t out = 0u;
 #pragma unroll
@@ -933,27 +1025,20 @@ extract_vertices_kernel(const uint32_t* __restrict__ bitmap,
                         uint64_t* __restrict__ vertices)
 {
     const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    // scene of this word: last s with scene_word_begin[s] <= w (all lanes take part in the ballot)
+    const int32_t lo = find_segment_rising([&](int32_t i) { return scene_word_begin[i]; }, nscenes, w);
     if (w >= nwords)
         return;
     uint32_t word = bitmap[w];
     if (word == 0u)
         return;
-    // scene of this word: last s with scene_word_begin[s] <= w
-    int32_t lo = 0, hi = nscenes - 1;
-    while (lo < hi)
-    {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (scene_word_begin[mid] <= w)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
     const SceneDev& s = scenes[lo];
     const uint64_t local = w - scene_word_begin[lo];
     const uint64_t R1 = (uint64_t(1) << s.level) + 1;
-    const uint64_t row = local / s.row_words;
+    const uint64_t row = div_u64_u32(local, s.row_words);
     const uint32_t xw = uint32_t(local - row * s.row_words);
-    const uint32_t y = uint32_t(row % R1), z = uint32_t(row / R1);
+    const uint64_t zq = div_u64_u32(row, uint32_t(R1));
+    const uint32_t y = uint32_t(row - zq * R1), z = uint32_t(zq);
     uint32_t pos = prefix[w];
     while (word != 0u)
     {
@@ -990,12 +1075,12 @@ launch_extract_vertices(const uint32_t* bitmap,
 // same float product.
 // ===========================================================================
 
+// float(int64_t(r)) for r = roundf(...) (integral, +-inf or NaN) with the
+// saturating float -> int64 conversion: NaN -> 0, +-inf and |r| >= 2^63 -> +-2^63.
 __device__ __forceinline__ float
-i64_to_f32(int64_t v)
+int64_round_trip(float r)
 {
-    if (v >= INT32_MIN && v <= INT32_MAX)
-        return float(int32_t(v));
-    return float(v);
+    return (r != r) ? 0.0f : fminf(fmaxf(r, -9223372036854775808.0f), 9223372036854775808.0f);
 }
 
 __device__ __forceinline__ float
@@ -1038,17 +1123,24 @@ field_value(const SceneDev& s, uint64_t key)
             if (!(rx >= 1.0f && rx <= Wm2 && ry >= 1.0f && ry <= Hm2))
                 continue;
         }
-        const int64_t y0 = f2i64_sat(roundf(py + -0.5f));
-        const int64_t x0 = f2i64_sat(roundf(px + -0.5f));
-        const bool y0in = y0 >= 0 && y0 < H, y1in = y0 >= -1 && y0 < H - 1;
-        const bool x0in = x0 >= 0 && x0 < W, x1in = x0 >= -1 && x0 < W - 1;
+        // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
+        // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
+        // range tests are exact in float for integral values and H, W < 2^24.
+        const float fy = int64_round_trip(roundf(py + -0.5f));
+        const float fx = int64_round_trip(roundf(px + -0.5f));
+        const bool y0in = fy >= 0.0f && fy < fH, y1in = fy >= -1.0f && fy < fH - 1.0f;
+        const bool x0in = fx >= 0.0f && fx < fW, x1in = fx >= -1.0f && fx < fW - 1.0f;
+        // in-plane offsets, exact whenever the sample is read (H * W < 2^31); wrapped otherwise
+        const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -1.0f), fH)));
+        const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -1.0f), fW)));
+        const uint32_t o00 = iy * uint32_t(W) + ix;
         const int64_t base = b * plane;
-        const float v00 = (y0in && x0in) ? mask_at(s.masks, s.mask_dtype, base + y0 * W + x0) : pad;
-        const float v10 = (y1in && x0in) ? mask_at(s.masks, s.mask_dtype, base + (y0 + 1) * W + x0) : pad;
-        const float v01 = (y0in && x1in) ? mask_at(s.masks, s.mask_dtype, base + y0 * W + x0 + 1) : pad;
-        const float v11 = (y1in && x1in) ? mask_at(s.masks, s.mask_dtype, base + (y0 + 1) * W + x0 + 1) : pad;
-        const float ty = py - i64_to_f32(y0);
-        const float tx = px - i64_to_f32(x0);
+        const float v00 = (y0in && x0in) ? mask_at(s.masks, s.mask_dtype, base + o00) : pad;
+        const float v10 = (y1in && x0in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + uint32_t(W))) : pad;
+        const float v01 = (y0in && x1in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + 1u)) : pad;
+        const float v11 = (y1in && x1in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + uint32_t(W) + 1u)) : pad;
+        const float ty = py - fy;
+        const float tx = px - fx;
         const float l0 = lerp_ref(v00, v10, ty);
         const float l1 = lerp_ref(v01, v11, ty);
         value *= lerp_ref(l0, l1, tx);
