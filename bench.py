@@ -135,6 +135,70 @@ def roofline_from(levels: list[dict]) -> dict:
             "bytes_per_launch": int(tot_b / max(n, 1)), "avg_launch_ms": round(tot_ms / max(n, 1), 4)}
 
 
+def serve(rate: float, seconds: float, pool_size: int, workers: int, dev: torch.device) -> dict:
+    """Open-loop Poisson request stream through the continuous-batching server.
+
+    Requests draw scenes from a pool of mixed-workload scenes with levels 6..10
+    (masks resident in HBM); every returned mesh is compared bit for bit with
+    the plain per-scene result of its scene."""
+    from torchhull_amd import serving
+
+    log(f"serving: rendering a pool of {pool_size} scenes (levels 6..10) ...")
+    pool = synthetic.mixed_scenes(pool_size, seed=4000, device=dev, max_level=10)
+    plain = [th.visual_hull(**scene_args(sc, blur(sc)), unique_verts=True) for sc in pool]
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(4001)
+    n = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    picks = rng.integers(0, pool_size, size=n)
+    lat, batch_sizes = [], []
+    mismatches = [0]
+    lock = __import__("threading").Lock()
+
+    def done(fut, k, t_sched):
+        if fut.exception() is not None:
+            with lock:
+                mismatches[0] += 1
+            return
+        r = fut.result()
+        pv, pf = plain[k]
+        ok = torch.equal(r.faces, pf) and torch.equal(r.verts.view(torch.int32), pv.view(torch.int32))
+        with lock:
+            lat.append(r.finished - t_sched)
+            batch_sizes.append(r.batch_size)
+            if not ok:
+                mismatches[0] += 1
+
+    log(f"serving: {n} requests at {rate:.0f} req/s for {seconds:.0f} s, {workers} workers ...")
+    with serving.HullServer(dev, workers=workers) as srv:
+        # warm-up: one request per worker, not counted
+        for f in [srv.submit(serving.HullRequest(**scene_args(pool[i % pool_size], pool[i % pool_size].masks),
+                                                 blur=pool[i % pool_size].blur)) for i in range(workers)]:
+            f.result()
+        t0 = time.perf_counter()
+        futs = []
+        for i in range(n):
+            t_sched = t0 + float(arrivals[i])
+            dt = t_sched - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            sc = pool[int(picks[i])]
+            fut = srv.submit(serving.HullRequest(**scene_args(sc, sc.masks), blur=sc.blur))
+            fut.add_done_callback(lambda f, k=int(picks[i]), ts=t_sched: done(f, k, ts))
+            futs.append(fut)
+            if (i + 1) % max(1, n // 6) == 0:
+                log(f"  submitted {i + 1}/{n}")
+        for f in futs:
+            f.result()
+        t_end = time.perf_counter()
+    lat_ms = np.array(lat) * 1e3
+    return {"rate_rps": rate, "seconds": seconds, "requests": n, "workers": workers,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "max_ms": round(float(lat_ms.max()), 3), "hulls_per_s": round(n / (t_end - t0), 2),
+            "mean_batch": round(float(np.mean(batch_sizes)), 2), "mismatches_vs_plain": mismatches[0],
+            "pool": f"{pool_size} mixed scenes, levels 6-10, 4-48 cameras, 256/512/1024 px, partial p=0.3, blur k7"}
+
+
 def cpu_baseline(scenes, gpu_meshes, budget_s: float) -> dict:
     """The CPU oracle (C++ restatement of the reference path, OpenMP) on the first scenes until the budget."""
     from oracle import oracle
@@ -174,8 +238,13 @@ def main() -> None:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the level-10 carving measurement of config 2")
     ap.add_argument("--batched-only", action="store_true", help="skip the per-scene path (for profiling runs)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson request rate of the serving run")
+    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-pool", type=int, default=32)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="first run one float32 tensor copy of this many MiB (HBM counter calibration)")
+                    help="first run one elementwise int32 kernel (bitwise_not) that reads and writes this many MiB "
+                         "(HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -186,12 +255,12 @@ def main() -> None:
 
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * 2**20 // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
+        src = torch.ones(n, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.bitwise_not(src, out=dst)  # one elementwise kernel: n * 4 bytes read, n * 4 bytes written
         torch.cuda.synchronize()
-        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        log(f"calibration kernel: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
         del src, dst
 
     log("rendering mixed workload scenes on the GPU ...")
@@ -252,6 +321,9 @@ def main() -> None:
             "camera_tests_per_s": round(last["camera_tests"] / (last["ms"] * 1e-3), 1),
             "hull_ms": round(1e3 * t2 / 3, 3)}
         del m2, sc2
+
+    if args.serve_seconds > 0:
+        result["serving"] = serve(args.serve_rate, args.serve_seconds, args.serve_pool, args.serve_workers, dev)
 
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle) ...")

@@ -51,3 +51,26 @@ def test_api_rejects_cpu_tensors_like_the_reference():
         th.gaussian_blur(m, 3, 1.0)
     with pytest.raises(RuntimeError, match="No backend implementation available"):
         th.marching_cubes(torch.zeros((1, 4, 4, 4)), 0.0)
+
+
+def test_serving_footprint_and_device_checks():
+    import torch
+
+    from torchhull_amd import serving
+
+    r = serving.HullRequest(masks=torch.zeros(8, 256, 256, 1), transforms=torch.zeros(8, 4, 4), level=7,
+                            cube_corner_bfl=(-1.0, -1.0, -1.0), cube_length=2.0, masks_partial=False,
+                            transforms_convention="opengl", blur=(7, 1.5))
+    fp = serving.footprint_bytes(r)
+    # blur output + padded integral images alone
+    assert fp > 8 * 256 * 256 * 4 * 2 + 8 * 257 * 257 * 4
+    r10 = dataclasses_replace(r, level=10)
+    assert serving.footprint_bytes(r10) > 32 * fp
+    with pytest.raises(RuntimeError):
+        serving.HullServer("cpu")
+
+
+def dataclasses_replace(obj, **kw):
+    import dataclasses
+
+    return dataclasses.replace(obj, **kw)

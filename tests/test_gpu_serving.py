@@ -1,0 +1,56 @@
+"""Serving loop: every mesh returned in a concurrent mix equals the mesh of the scene computed alone."""
+
+import pytest
+import torch
+
+import torchhull_amd as th
+from torchhull_amd import serving, synthetic
+
+from gpu_util import DEV
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(n, seed):
+    scenes = synthetic.mixed_scenes(n, seed=seed, device=DEV, max_level=7, resolution_cap=256)
+    reqs = [serving.HullRequest(masks=sc.masks, transforms=sc.transforms, level=sc.level,
+                                cube_corner_bfl=sc.cube_corner_bfl, cube_length=sc.cube_length,
+                                masks_partial=sc.masks_partial, transforms_convention=sc.transforms_convention,
+                                blur=sc.blur)
+            for sc in scenes]
+    return reqs
+
+
+def _alone(r):
+    m = r.masks if r.blur is None else th.gaussian_blur(r.masks, r.blur[0], r.blur[1], sparse=True,
+                                                        dtype=torch.float32)
+    return th.visual_hull(m, r.transforms, r.level, r.cube_corner_bfl, r.cube_length, r.masks_partial,
+                          r.transforms_convention, unique_verts=r.unique_verts)
+
+
+def _same(a, b):
+    return torch.equal(a[1], b[1]) and torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))
+
+
+def test_results_in_the_mix_equal_alone():
+    reqs = _requests(12, 5100)
+    reqs[3].unique_verts = False  # a soup request in the same stream of work
+    alone = [_alone(r) for r in reqs]
+    with serving.HullServer(DEV, workers=3, max_batch=4) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = [f.result(timeout=120) for f in futs]
+    assert srv.admitted == len(reqs)
+    assert srv.batches >= 3  # batched, more than one batch
+    assert max(r.batch_size for r in res) > 1
+    for r, a in zip(res, alone):
+        assert _same((r.verts, r.faces), a)
+
+
+def test_budget_smaller_than_one_request_runs_requests_alone():
+    reqs = _requests(4, 5200)
+    alone = [_alone(r) for r in reqs]
+    with serving.HullServer(DEV, workers=2, hbm_budget=1) as srv:
+        res = [f.result(timeout=120) for f in [srv.submit(r) for r in reqs]]
+    assert all(r.batch_size == 1 for r in res)
+    for r, a in zip(res, alone):
+        assert _same((r.verts, r.faces), a)

@@ -55,10 +55,16 @@ inline constexpr const char* kMcCaseText[256] = {
     "23828aa89", "9a2092", "23828a0181a8", "1a2", "138918", "091", "038", "",
 };
 
+// Edge e of a cell: offset of its lower endpoint in the cell (bit 0 x, bit 1 y,
+// bit 2 z) and its axis (0 x, 1 y, 2 z).
+constexpr uint8_t kMcEdgeLow[12] = { 0b000, 0b001, 0b100, 0b000, 0b010, 0b011, 0b110, 0b010, 0b000, 0b001, 0b101, 0b100 };
+constexpr uint8_t kMcEdgeAxis[12] = { 0, 2, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1 };
+
 struct McCaseTable
 {
     int8_t edges[256][16]; // edge ids per case, -1 terminated
     int8_t count[256];     // number of triangle vertices per case (multiple of 3)
+    uint8_t lows[256];     // bit o: cell offset o is the lower endpoint of an edge of the case
 };
 
 constexpr int
@@ -79,6 +85,11 @@ mc_build_case_table()
             t.edges[c][n++] = static_cast<int8_t>(mc_hex_digit(*p));
         }
         t.count[c] = static_cast<int8_t>(n);
+        t.lows[c] = 0;
+        for (int k = 0; k < n; ++k)
+        {
+            t.lows[c] = static_cast<uint8_t>(t.lows[c] | (1u << kMcEdgeLow[t.edges[c][k]]));
+        }
         for (int k = n; k < 16; ++k)
         {
             t.edges[c][k] = -1;

@@ -29,9 +29,15 @@ __constant__ McCaseTable c_cases = mc_build_case_table();
 
 // Case corner b -> offset (bit0 x, bit1 y, bit2 z) in the cell.
 __constant__ uint8_t c_corner_off[8] = { 0b000, 0b001, 0b101, 0b100, 0b010, 0b011, 0b111, 0b110 };
+// the same table for compile-time indexing in unrolled loops
+constexpr uint8_t c_corner_off_host[8] = { 0b000, 0b001, 0b101, 0b100, 0b010, 0b011, 0b111, 0b110 };
 // Edge e -> lower endpoint offset (bits as above) and axis (0 x, 1 y, 2 z).
-__constant__ uint8_t c_edge_low[12] = { 0b000, 0b001, 0b100, 0b000, 0b010, 0b011, 0b110, 0b010, 0b000, 0b001, 0b101, 0b100 };
-__constant__ uint8_t c_edge_axis[12] = { 0, 2, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1 };
+__constant__ uint8_t c_edge_low[12] = { kMcEdgeLow[0], kMcEdgeLow[1], kMcEdgeLow[2],  kMcEdgeLow[3],
+                                        kMcEdgeLow[4], kMcEdgeLow[5], kMcEdgeLow[6],  kMcEdgeLow[7],
+                                        kMcEdgeLow[8], kMcEdgeLow[9], kMcEdgeLow[10], kMcEdgeLow[11] };
+__constant__ uint8_t c_edge_axis[12] = { kMcEdgeAxis[0], kMcEdgeAxis[1], kMcEdgeAxis[2],  kMcEdgeAxis[3],
+                                         kMcEdgeAxis[4], kMcEdgeAxis[5], kMcEdgeAxis[6],  kMcEdgeAxis[7],
+                                         kMcEdgeAxis[8], kMcEdgeAxis[9], kMcEdgeAxis[10], kMcEdgeAxis[11] };
 
 constexpr uint32_t kInfoComplete = 1u << 8;
 constexpr int kInfoSignShift = 9;
@@ -52,6 +58,31 @@ struct BitmapLookup
         pos = prefix[w] + __popc(word & ((1u << bit) - 1u));
         return true;
     }
+
+    // Split lookup for batching: probe() issues the bitmap read, present()
+    // and rank() use it, so that the probes of several points are in flight
+    // together.
+    struct Probe
+    {
+        uint64_t w;
+        uint32_t word, bit;
+    };
+    __device__ __forceinline__ Probe
+    probe(const McGrid& g, uint32_t x, uint32_t y, uint32_t z) const
+    {
+        const uint64_t w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
+        return Probe{ w, bitmap[w], x & 31u };
+    }
+    __device__ __forceinline__ bool
+    present(const Probe& p) const
+    {
+        return ((p.word >> p.bit) & 1u) != 0u;
+    }
+    __device__ __forceinline__ uint32_t
+    rank(const Probe& p) const
+    {
+        return prefix[p.w] + __popc(p.word & ((1u << p.bit) - 1u));
+    }
 };
 
 struct DenseLookup
@@ -61,6 +92,26 @@ struct DenseLookup
     {
         pos = x + g.dims[0] * (y + g.dims[1] * z);
         return true;
+    }
+
+    struct Probe
+    {
+        uint32_t pos;
+    };
+    __device__ __forceinline__ Probe
+    probe(const McGrid& g, uint32_t x, uint32_t y, uint32_t z) const
+    {
+        return Probe{ x + g.dims[0] * (y + g.dims[1] * z) };
+    }
+    __device__ __forceinline__ bool
+    present(const Probe&) const
+    {
+        return true;
+    }
+    __device__ __forceinline__ uint32_t
+    rank(const Probe& p) const
+    {
+        return p.pos;
     }
 };
 
@@ -158,50 +209,50 @@ mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* _
     const V iso = static_cast<V>(p.iso);
     const V vg = value_at<V>(p, uint32_t(i));
     const bool below_g = vg < iso;
+    // the cell's other seven corners, case-corner order 1, 4, 3 (the +x, +y, +z neighbours), then 2, 5, 6, 7
+    constexpr uint32_t corner[7] = { 1u, 4u, 3u, 2u, 5u, 6u, 7u };
+    const bool inx = r.x + 1 < g.dims[0], iny = r.y + 1 < g.dims[1], inz = r.z + 1 < g.dims[2];
+    typename Lookup::Probe pr[7];
+    bool in[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+    {
+        const uint32_t o = c_corner_off_host[corner[k]];
+        in[k] = (!(o & 1u) || inx) && (!((o >> 1) & 1u) || iny) && (!((o >> 2) & 1u) || inz);
+        if (in[k])
+            pr[k] = lk.probe(g, r.x + (o & 1u), r.y + ((o >> 1) & 1u), r.z + ((o >> 2) & 1u));
+    }
+    bool found[7];
+    bool complete = inx && iny && inz; // MC:199
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+    {
+        found[k] = in[k] && lk.present(pr[k]);
+        complete = complete && found[k];
+    }
+    // values: the axis neighbours when present, the other corners of a complete cell
+    uint32_t pos[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        if (k < 3 ? found[k] : complete)
+            pos[k] = lk.rank(pr[k]);
+    bool below[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        below[k] = (k < 3 ? found[k] : complete) && value_at<V>(p, pos[k]) < iso;
     uint32_t sc = 0u;
-    uint32_t ci = below_g ? 1u : 0u;
-    bool complete = (r.x + 1 < g.dims[0]) && (r.y + 1 < g.dims[1]) && (r.z + 1 < g.dims[2]); // MC:199
-    // neighbours along +x, +y, +z (case corners 1, 4, 3)
-    const uint32_t nb_corner[3] = { 1u, 4u, 3u };
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-    {
-        const uint32_t nx = r.x + (d == 0), ny = r.y + (d == 1), nz = r.z + (d == 2);
-        uint32_t pos;
-        bool found = false;
-        if (nx < g.dims[0] && ny < g.dims[1] && nz < g.dims[2])
-            found = lk.find(g, nx, ny, nz, pos);
-        if (found)
-        {
-            const bool below = value_at<V>(p, pos) < iso;
-            if (below != below_g)
-                sc |= 1u << d;
-            if (below)
-                ci |= 1u << nb_corner[d];
-        }
-        else
-            complete = false;
-    }
+        if (found[d] && below[d] != below_g)
+            sc |= 1u << d;
+    uint32_t ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
     if (complete)
     {
-        // remaining case corners 2 (1,0,1), 5 (1,1,0), 6 (1,1,1), 7 (0,1,1)
-        const uint32_t rest[4] = { 2u, 5u, 6u, 7u };
+        ci = below_g ? 1u : 0u;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-        {
-            const uint32_t o = c_corner_off[rest[k]];
-            uint32_t pos;
-            if (!lk.find(g, r.x + (o & 1u), r.y + ((o >> 1) & 1u), r.z + ((o >> 2) & 1u), pos))
-            {
-                complete = false;
-                break;
-            }
-            if (value_at<V>(p, pos) < iso)
-                ci |= 1u << rest[k];
-        }
+        for (int k = 0; k < 7; ++k)
+            ci |= below[k] ? (1u << corner[k]) : 0u;
     }
-    if (!complete)
-        ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
     info[i] = ci | (complete ? kInfoComplete : 0u) | (sc << kInfoSignShift);
     ntv[i] = complete ? uint32_t(c_cases.count[ci]) : 0u;
 }
@@ -275,6 +326,17 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
     const McGrid& g = p.grids[r.scene];
     const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
     const uint32_t p1[3] = { r.x, r.y, r.z };
+    // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
+    typename Lookup::Probe pr[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (m & (1u << d))
+            pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
+    float v2[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (m & (1u << d))
+            v2[d] = float(value_at<V>(p, lk.rank(pr[d])));
     uint32_t out = vbase[i];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -282,11 +344,8 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
         if (!(m & (1u << d)))
             continue;
         const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
-        uint32_t pos = 0;
-        lk.find(g, p2[0], p2[1], p2[2], pos); // present: the edge has a sign change
-        const float v2 = float(value_at<V>(p, pos));
         float v[3];
-        interpolate_vertex(p.iso, p1, p2, v1, v2, v);
+        interpolate_vertex(p.iso, p1, p2, v1, v2[d], v);
         finish_vertex(g, v, verts + 3 * uint64_t(out));
         ++out;
     }
@@ -325,6 +384,51 @@ mc_faces_kernel(McPlan p,
     const uint32_t t0 = tbase[i];
     const uint32_t ts = scene_tstart[r.scene];
     const uint32_t vs = unique ? scene_vstart[r.scene] : 0u;
+    if (unique)
+    {
+        // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
+        // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
+        const uint32_t lows = c_cases.lows[ci];
+        typename Lookup::Probe pr[8];
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
+        uint32_t pos[8];
+        pos[0] = uint32_t(i);
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pos[o] = lk.rank(pr[o]);
+        uint32_t vb[8], em[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (lows & (1u << o))
+            {
+                vb[o] = vbase[pos[o]];
+                em[o] = emask[pos[o]];
+            }
+        for (int k = 0; k < count; ++k)
+        {
+            const int e = c_cases.edges[ci][k];
+            const uint32_t lo = c_edge_low[e];
+            const int axis = c_edge_axis[e];
+            uint32_t b = 0u, m = 0u;
+#pragma unroll
+            for (int o = 0; o < 8; ++o)
+                if (lo == uint32_t(o))
+                {
+                    b = vb[o];
+                    m = em[o];
+                }
+            const int64_t idx = int64_t(b + __popc(m & ((1u << axis) - 1u)) - vs);
+            int col = k % 3;
+            if (g.flip && col != 0)
+                col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
+            faces[3 * uint64_t((t0 + uint32_t(k)) / 3u) + col] = idx;
+        }
+        return;
+    }
     for (int k = 0; k < count; ++k)
     {
         const int e = c_cases.edges[ci][k];

@@ -1,0 +1,210 @@
+"""Request loop for visual hulls on one GPU: footprint admission and continuous batching.
+
+A HullServer owns `workers` host threads, each with its own HIP stream and its
+own library context (whose pooled device memory stays resident between
+batches).  A request is one scene (masks already in device memory, optional
+sparse blur, visual-hull arguments).  Whenever a worker is free it admits the
+waiting requests in arrival order while their estimated HBM footprint fits in
+the budget still free, and runs them as one batched pipeline (one blur launch
+per pass, one carving launch per octree level for all admitted scenes).
+Requests that arrive while other batches are on the GPU are admitted by the
+next free worker, so batches overlap on the device; memory is returned to the
+budget when a batch completes.
+
+Every result equals the result of the plain per-scene calls on the same
+inputs: scenes of one batch never interact (tests/test_gpu_serving.py checks
+it bit for bit).
+"""
+
+from __future__ import annotations
+
+import collections
+import concurrent.futures
+import dataclasses
+import threading
+import time
+from typing import Sequence
+
+import torch
+
+from . import api
+
+
+@dataclasses.dataclass
+class HullRequest:
+    """One visual hull to compute: the keyword arguments of visual_hull plus an optional blur.
+
+    masks [B,H,W,1] and transforms [B,4,4] on the server's device; blur =
+    (kernel_size, sigma) applies the sparse gaussian_blur (float32) first."""
+
+    masks: torch.Tensor
+    transforms: torch.Tensor
+    level: int
+    cube_corner_bfl: Sequence[float]
+    cube_length: float
+    masks_partial: bool
+    transforms_convention: str
+    blur: tuple[int, float] | None = None
+    unique_verts: bool = True
+
+
+@dataclasses.dataclass
+class HullResult:
+    verts: torch.Tensor
+    faces: torch.Tensor
+    submitted: float   # time.perf_counter() at submit
+    started: float     # when its batch was admitted
+    finished: float    # when its mesh was ready in device memory
+    batch_size: int
+
+
+def footprint_bytes(req: HullRequest) -> int:
+    """Estimated device memory of one request beyond its resident inputs.
+
+    Blur output and the padded integral images (4 bytes per pixel each), the
+    leaf-cell and vertex bitmaps of the target level plus the vertex prefix
+    (one 32-bit word per 32 grid points), and 600 bytes per expected surface
+    cell (about 3 R^2 at resolution R) for the octree queues, the field and the
+    marching-cubes arrays and output."""
+    B, H, W = int(req.masks.size(0)), int(req.masks.size(1)), int(req.masks.size(2))
+    R = 1 << int(req.level)
+    images = B * H * W * 4 * (2 if req.blur is not None else 1) + B * (H + 1) * (W + 1) * 4
+    cell_bitmap = R * R * ((R + 31) // 32) * 4
+    vert_bitmap = 2 * (R + 1) * (R + 1) * ((R + 32) // 32) * 4
+    surface = 600 * 3 * R * R
+    return images + cell_bitmap + vert_bitmap + surface
+
+
+class HullServer:
+    """Continuous-batching visual-hull server on one GPU (see the module docstring).
+
+    hbm_budget: bytes of device memory the admitted requests may use together
+    (footprint_bytes each); a request larger than the whole budget is still
+    admitted, alone, when nothing else is in flight.  max_batch bounds the
+    scenes of one batch."""
+
+    def __init__(self, device: torch.device | str | None = None, workers: int = 3, hbm_budget: int = 24 << 30,
+                 max_batch: int = 32) -> None:
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if self.device.type != "cuda":
+            raise RuntimeError("HullServer runs on a GPU device")
+        if workers < 1 or max_batch < 1:
+            raise ValueError("workers and max_batch must be positive")
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self._pending: collections.deque = collections.deque()
+        self._cv = threading.Condition()
+        self._in_use = 0
+        self._in_flight = 0
+        self._closed = False
+        self.batches = 0
+        self.admitted = 0
+        self._threads = [threading.Thread(target=self._worker, name=f"hull-worker-{i}", daemon=True)
+                         for i in range(workers)]
+        for t in self._threads:
+            t.start()
+
+    # -- public --------------------------------------------------------------------------------------------------
+
+    def submit(self, req: HullRequest) -> concurrent.futures.Future:
+        if req.masks.device != self.device or req.transforms.device != self.device:
+            raise RuntimeError("request tensors must be on the server's device")
+        fut: concurrent.futures.Future = concurrent.futures.Future()
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server is closed")
+            self._pending.append((req, fut, time.perf_counter(), footprint_bytes(req)))
+            self._cv.notify()
+        return fut
+
+    def close(self) -> None:
+        """Finishes the queued requests, then stops the workers."""
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join()
+
+    def __enter__(self) -> "HullServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    # -- workers -------------------------------------------------------------------------------------------------
+
+    def _admit(self) -> list | None:
+        """Takes the next batch off the queue (caller holds the lock); None when closed and drained."""
+        while True:
+            if self._pending:
+                batch, fp = [], 0
+                while self._pending and len(batch) < self.max_batch:
+                    f = self._pending[0][3]
+                    fits = self._in_use + fp + f <= self.hbm_budget
+                    alone = not batch and self._in_flight == 0  # an oversized request runs by itself
+                    if not (fits or alone):
+                        break
+                    batch.append(self._pending.popleft())
+                    fp += f
+                if batch:
+                    self._in_use += fp
+                    self._in_flight += 1
+                    return batch
+            elif self._closed:
+                return None
+            self._cv.wait()
+
+    def _worker(self) -> None:
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(self.device)
+        while True:
+            with self._cv:
+                batch = self._admit()
+            if batch is None:
+                return
+            started = time.perf_counter()
+            fp = sum(item[3] for item in batch)
+            try:
+                with torch.cuda.stream(stream):
+                    meshes = self._run(batch)
+                    stream.synchronize()
+                finished = time.perf_counter()
+                for (req, fut, sub, _), (v, f) in zip(batch, meshes):
+                    fut.set_result(HullResult(v, f, sub, started, finished, len(batch)))
+            except BaseException as e:  # noqa: BLE001 - delivered to every waiter of the batch
+                for _, fut, _, _ in batch:
+                    if not fut.done():
+                        fut.set_exception(e)
+            finally:
+                with self._cv:
+                    self._in_use -= fp
+                    self._in_flight -= 1
+                    self.batches += 1
+                    self.admitted += len(batch)
+                    self._cv.notify_all()
+
+    @staticmethod
+    def _run(batch: list) -> list[tuple[torch.Tensor, torch.Tensor]]:
+        reqs = [item[0] for item in batch]
+        masks = [r.masks for r in reqs]
+        # one blur launch per (kernel_size, sigma, input dtype) group
+        groups: dict = {}
+        for i, r in enumerate(reqs):
+            if r.blur is not None:
+                groups.setdefault((int(r.blur[0]), float(r.blur[1]), r.masks.dtype), []).append(i)
+        for (k, sigma, _), idx in groups.items():
+            out = api.gaussian_blur_batch([reqs[i].masks for i in idx], k, sigma, sparse=True, dtype=torch.float32)
+            for i, o in zip(idx, out):
+                masks[i] = o
+        results: list = [None] * len(reqs)
+        for unique in (True, False):
+            idx = [i for i, r in enumerate(reqs) if r.unique_verts == unique]
+            if not idx:
+                continue
+            scenes = [dict(masks=masks[i], transforms=reqs[i].transforms, level=reqs[i].level,
+                           cube_corner_bfl=reqs[i].cube_corner_bfl, cube_length=reqs[i].cube_length,
+                           masks_partial=reqs[i].masks_partial, transforms_convention=reqs[i].transforms_convention)
+                      for i in idx]
+            for i, mesh in zip(idx, api.visual_hull_batch(scenes, unique_verts=unique)):
+                results[i] = mesh
+        return results
