@@ -20,6 +20,7 @@ import math
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -153,21 +154,24 @@ def serve(rate: float, seconds: float, pool_size: int, workers: int, dev: torch.
     picks = rng.integers(0, pool_size, size=n)
     lat, batch_sizes = [], []
     mismatches = [0]
-    lock = __import__("threading").Lock()
+    completed = [0]
+    cv = threading.Condition()
 
+    # futures are not kept: each mesh is checked and released as soon as it is done
     def done(fut, k, t_sched):
-        if fut.exception() is not None:
-            with lock:
-                mismatches[0] += 1
-            return
-        r = fut.result()
-        pv, pf = plain[k]
-        ok = torch.equal(r.faces, pf) and torch.equal(r.verts.view(torch.int32), pv.view(torch.int32))
-        with lock:
-            lat.append(r.finished - t_sched)
-            batch_sizes.append(r.batch_size)
+        ok, r = False, None
+        if fut.exception() is None:
+            r = fut.result()
+            pv, pf = plain[k]
+            ok = torch.equal(r.faces, pf) and torch.equal(r.verts.view(torch.int32), pv.view(torch.int32))
+        with cv:
+            if r is not None:
+                lat.append(r.finished - t_sched)
+                batch_sizes.append(r.batch_size)
             if not ok:
                 mismatches[0] += 1
+            completed[0] += 1
+            cv.notify_all()
 
     log(f"serving: {n} requests at {rate:.0f} req/s for {seconds:.0f} s, {workers} workers ...")
     with serving.HullServer(dev, workers=workers) as srv:
@@ -176,7 +180,6 @@ def serve(rate: float, seconds: float, pool_size: int, workers: int, dev: torch.
                                                  blur=pool[i % pool_size].blur)) for i in range(workers)]:
             f.result()
         t0 = time.perf_counter()
-        futs = []
         for i in range(n):
             t_sched = t0 + float(arrivals[i])
             dt = t_sched - time.perf_counter()
@@ -185,11 +188,11 @@ def serve(rate: float, seconds: float, pool_size: int, workers: int, dev: torch.
             sc = pool[int(picks[i])]
             fut = srv.submit(serving.HullRequest(**scene_args(sc, sc.masks), blur=sc.blur))
             fut.add_done_callback(lambda f, k=int(picks[i]), ts=t_sched: done(f, k, ts))
-            futs.append(fut)
+            del fut
             if (i + 1) % max(1, n // 6) == 0:
                 log(f"  submitted {i + 1}/{n}")
-        for f in futs:
-            f.result()
+        with cv:
+            cv.wait_for(lambda: completed[0] >= n)
         t_end = time.perf_counter()
     lat_ms = np.array(lat) * 1e3
     return {"rate_rps": rate, "seconds": seconds, "requests": n, "workers": workers,

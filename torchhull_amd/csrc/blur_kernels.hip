@@ -82,6 +82,27 @@ convert_in(Ti v)
         return OutOps<To>::from_float(float(v));
 }
 
+// Element read through a global pointer; binary16 values are read as their bits.
+template <typename T>
+__device__ __forceinline__ T
+ld(gptr<const T> p, int64_t i)
+{
+    if constexpr (std::is_same_v<T, __half>)
+        return __ushort_as_half(((gptr<const unsigned short>)p)[i]);
+    else
+        return p[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void
+st(gptr<T> p, int64_t i, T v)
+{
+    if constexpr (std::is_same_v<T, __half>)
+        ((gptr<unsigned short>)p)[i] = __half_as_ushort(v);
+    else
+        p[i] = v;
+}
+
 // Pixel comparison in the input type (GB:140); half compares as float.
 template <typename Ti>
 __device__ __forceinline__ bool
@@ -115,25 +136,22 @@ struct BlurJob
 {
     const void* in;
     void* out;
-    uint8_t* flags;      // B * TH * TW edge-tile flags, then B * TH * TW band-tile flags (sparse)
+    uint8_t* flags;      // B * TH * TW edge-tile flags (sparse)
     int32_t B, H, W;
     int32_t TH, TW;      // tile grid (W / ts + 1, H / ts + 1)
     uint32_t bx, by;     // 64 x RB pixel blocks per image (x, y)
-    uint32_t tb;         // tile blocks per image (256 tiles each) of the dilation pass
-    uint32_t pad_;
     uint64_t p_begin;    // first pixel block of this job
-    uint64_t t_begin;    // first tile block of this job
 };
 
 __device__ __forceinline__ int32_t
-find_blur_job(const BlurJob* jobs, int32_t n, uint64_t blk, bool tiles)
+find_blur_job(const BlurJob* jobs, int32_t n, uint64_t blk)
 {
-    if (tiles)
-        return find_segment_uniform([&](int32_t i) { return jobs[i].t_begin; }, n, blk);
     return find_segment_uniform([&](int32_t i) { return jobs[i].p_begin; }, n, blk);
 }
 
-constexpr int kBlurBX = 64; // pixel block: 64 columns x RB rows, 256 threads
+constexpr int kBlurBX = 64;
+// edge-flag tiles around one pixel block: (64 / ts + 3) x (RB / ts + 3) for ts >= 1, RB <= 64
+constexpr int kBlurMaxFlagTiles = (kBlurBX + 3) * (64 + 3); // pixel block: 64 columns x RB rows, 256 threads
 
 // Edge pixels (GB:116-142) marked into their tile's flag.  Every writer
 // stores the same value 1, so the plain byte stores need no atomics; flags
@@ -145,7 +163,7 @@ blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t 
 {
     constexpr int R = RB / 4;
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -156,18 +174,18 @@ blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t 
     const int32_t yb = by * RB + int32_t(threadIdx.x >> 6) * R;
     if (x >= W || yb >= H)
         return;
-    const Ti* __restrict__ I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W + x;
+    const auto I = as_global(static_cast<const Ti*>(j.in) + int64_t(b) * H * W + x);
     const bool has_right = x + 1 < W;
     Ti c[R + 1], rt[R];
 #pragma unroll
     for (int q = 0; q <= R; ++q)
     {
         const bool in = yb + q < H; // row yb + q exists (q == R: the row below the thread's last row)
-        c[q] = in ? I[int64_t(yb + q) * W] : (q > 0 ? c[q - 1] : Ti{});
+        c[q] = in ? ld(I, int64_t(yb + q) * W) : (q > 0 ? c[q - 1] : Ti{});
         if (q < R)
-            rt[q] = (in && has_right) ? I[int64_t(yb + q) * W + 1] : c[q];
+            rt[q] = (in && has_right) ? ld(I, int64_t(yb + q) * W + 1) : c[q];
     }
-    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW + x / ts;
+    const auto F = as_global(j.flags + int64_t(b) * j.TH * j.TW + x / ts);
 #pragma unroll
     for (int q = 0; q < R; ++q)
     {
@@ -186,7 +204,7 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
 {
     constexpr int R = RB / 4;
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -197,7 +215,7 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
     const int32_t yb = by * RB + int32_t(threadIdx.x >> 6) * R;
     if (x >= W || yb >= H)
         return;
-    const float* __restrict__ I = static_cast<const float*>(j.in) + int64_t(b) * H * W + x;
+    const auto I = as_global(static_cast<const float*>(j.in) + int64_t(b) * H * W + x);
     const bool has_right = x + 4 < W;
     float4 c[R + 1];
     float rt[R];
@@ -205,11 +223,11 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
     for (int q = 0; q <= R; ++q)
     {
         const bool in = yb + q < H;
-        c[q] = in ? *reinterpret_cast<const float4*>(I + int64_t(yb + q) * W) : (q > 0 ? c[q - 1] : float4{});
+        c[q] = in ? ld_float4(I + int64_t(yb + q) * W) : (q > 0 ? c[q - 1] : float4{});
         if (q < R)
             rt[q] = (in && has_right) ? I[int64_t(yb + q) * W + 4] : c[q].w;
     }
-    uint8_t* F = j.flags + int64_t(b) * j.TH * j.TW;
+    const auto F = as_global(j.flags + int64_t(b) * j.TH * j.TW);
 #pragma unroll
     for (int q = 0; q < R; ++q)
     {
@@ -222,7 +240,7 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
         const bool e3 = (rt[q] != u.w) || (d.w != u.w);
         if (e0 || e1 || e2 || e3)
         {
-            uint8_t* row = F + ((yb + q) / ts) * j.TW;
+            const auto row = F + ((yb + q) / ts) * j.TW;
             if (e0)
                 row[x / ts] = 1;
             if (e1)
@@ -233,28 +251,6 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
                 row[(x + 3) / ts] = 1;
         }
     }
-}
-
-// Band tiles: the 3x3 dilation of the edge tiles, clamped to the tile grid
-// (GB:154-176).  A pixel is a band pixel iff its tile is a band tile.
-__global__ __launch_bounds__(256) void
-blur_band_tiles_kernel(const BlurJob* __restrict__ jobs, int32_t njobs)
-{
-    const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, true)];
-    const uint64_t t = uint64_t(blk - j.t_begin) * 256 + threadIdx.x;
-    const uint64_t plane = uint64_t(j.TH) * j.TW;
-    if (t >= plane * j.B)
-        return;
-    const int32_t b = int32_t(t / plane);
-    const int32_t rem = int32_t(t - uint64_t(b) * plane);
-    const int32_t ty = rem / j.TW, tx = rem - ty * j.TW;
-    const uint8_t* E = j.flags + uint64_t(b) * plane;
-    uint8_t band = 0;
-    for (int32_t uy = max(ty - 1, 0); uy <= min(ty + 1, j.TH - 1); ++uy)
-        for (int32_t ux = max(tx - 1, 0); ux <= min(tx + 1, j.TW - 1); ++ux)
-            band |= E[uy * j.TW + ux];
-    j.flags[plane * j.B + t] = band;
 }
 
 // Band pixels of a 64 x RB block are convolved from an LDS tile with a halo
@@ -277,7 +273,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     constexpr int P = RB / 4; // pixels per thread
     const int32_t k = KT > 0 ? KT : k_rt;
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk, false)];
+    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -287,8 +283,8 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     const int32_t h = k / 2;
     const int32_t LW = kBlurBX + 2 * h, LH = RB + 2 * h;
     const int32_t x0 = bx * kBlurBX, y0 = by * RB;
-    const Ti* __restrict__ I = static_cast<const Ti*>(j.in) + int64_t(b) * H * W;
-    To* __restrict__ O = static_cast<To*>(j.out) + int64_t(b) * H * W;
+    const auto I = as_global(static_cast<const Ti*>(j.in) + int64_t(b) * H * W);
+    const auto O = as_global(static_cast<To*>(j.out) + int64_t(b) * H * W);
     const int32_t lx = int32_t(threadIdx.x & 63), ly0 = int32_t(threadIdx.x >> 6);
     const int32_t x = x0 + lx;
     const bool dense = !sparse && k > 1;
@@ -301,10 +297,38 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
         band = nq >= 32 ? 0xffffffffu : ((1u << nq) - 1u);
     else if (use_flags)
     {
-        const uint8_t* __restrict__ bt = j.flags + uint64_t(j.B + b) * j.TH * j.TW + x / ts;
+        // Band tiles (GB:154-176): a tile is a band tile when an edge tile lies in its 3x3 window,
+        // clamped to the tile grid.  The edge flags of the block's tiles and their neighbours go to
+        // LDS, the block's band tiles are derived there, and each pixel reads its tile's bit.
+        __shared__ uint8_t ef[kBlurMaxFlagTiles];
+        __shared__ uint8_t btl[kBlurMaxFlagTiles];
+        const int32_t tx_lo = x0 / ts, tx_hi = min(x0 + kBlurBX - 1, W - 1) / ts;
+        const int32_t ty_lo = y0 / ts, ty_hi = min(y0 + RB - 1, H - 1) / ts;
+        const int32_t ex_lo = max(tx_lo - 1, 0), ex_hi = min(tx_hi + 1, j.TW - 1);
+        const int32_t ey_lo = max(ty_lo - 1, 0), ey_hi = min(ty_hi + 1, j.TH - 1);
+        const int32_t ew = ex_hi - ex_lo + 1, eh = ey_hi - ey_lo + 1;
+        const auto E = as_global(j.flags + uint64_t(b) * j.TH * j.TW);
+        for (int32_t q = threadIdx.x; q < ew * eh; q += 256)
+        {
+            const int32_t r = q / ew;
+            ef[q] = E[int64_t(ey_lo + r) * j.TW + ex_lo + (q - r * ew)];
+        }
+        __syncthreads();
+        const int32_t bw = tx_hi - tx_lo + 1, bh = ty_hi - ty_lo + 1;
+        for (int32_t q = threadIdx.x; q < bw * bh; q += 256)
+        {
+            const int32_t r = q / bw;
+            const int32_t ty = ty_lo + r, tx = tx_lo + (q - r * bw);
+            uint8_t v = 0;
+            for (int32_t uy = max(ty - 1, 0); uy <= min(ty + 1, j.TH - 1); ++uy)
+                for (int32_t ux = max(tx - 1, 0); ux <= min(tx + 1, j.TW - 1); ++ux)
+                    v |= ef[(uy - ey_lo) * ew + (ux - ex_lo)];
+            btl[q] = v;
+        }
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < P; ++q)
-            if (q < nq && bt[((y0 + ly0 + 4 * q) / ts) * j.TW] != 0)
+            if (q < nq && btl[((y0 + ly0 + 4 * q) / ts - ty_lo) * bw + (x / ts - tx_lo)] != 0)
                 band |= 1u << q;
     }
     if (!__syncthreads_or(band != 0u))
@@ -313,11 +337,11 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
 #pragma unroll
         for (int q = 0; q < P; ++q)
             if (q < nq)
-                v[q] = convert_in<To>(I[int64_t(y0 + ly0 + 4 * q) * W + x]); // images.clone().to(dtype) (GB:444)
+                v[q] = convert_in<To>(ld(I, int64_t(y0 + ly0 + 4 * q) * W + x)); // images.clone().to(dtype) (GB:444)
 #pragma unroll
         for (int q = 0; q < P; ++q)
             if (q < nq)
-                O[int64_t(y0 + ly0 + 4 * q) * W + x] = v[q];
+                st(O, int64_t(y0 + ly0 + 4 * q) * W + x, v[q]);
         return;
     }
     // stage the block and its halo, converted to the output type (GB:221): wave w stages rows
@@ -333,7 +357,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
             {
                 const int32_t ty = ty0 + 4 * u;
                 if (ty < LH)
-                    v[u] = convert_in<To>(I[reflect_index(y0 + ty - h, H) * W + xx]);
+                    v[u] = convert_in<To>(ld(I, reflect_index(y0 + ty - h, H) * W + xx));
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -379,7 +403,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
                 }
             result = sparse ? OutOps<To>::div(acc, norm) : acc;
         }
-        O[int64_t(y) * W + x] = result;
+        st(O, int64_t(y) * W + x, result);
     }
 }
 
@@ -389,7 +413,7 @@ blur_tile_flag_bytes(int64_t B, int64_t H, int64_t W, int kernel_size)
     const int64_t ts = kernel_size / 2;
     if (ts <= 0)
         return 0;
-    return 2 * B * (H / ts + 1) * (W / ts + 1); // edge tiles, then band tiles
+    return B * (H / ts + 1) * (W / ts + 1); // edge tiles
 }
 
 size_t
@@ -409,7 +433,7 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
     bool vec = std::is_same_v<Ti, float>;
     for (auto& j : jobs)
         vec = vec && (j.W % 4 == 0) && (reinterpret_cast<uintptr_t>(j.in) % 16 == 0);
-    uint64_t pb = 0, tbk = 0, eb = 0, fbytes = 0;
+    uint64_t pb = 0, eb = 0, fbytes = 0;
     for (auto& j : jobs)
     {
         j.by = blocks_for(j.H, RB);
@@ -420,16 +444,13 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
         {
             j.TH = j.H / ts + 1;
             j.TW = j.W / ts + 1;
-            j.tb = uint32_t(blocks_for(int64_t(j.TH) * j.TW * j.B, 256));
-            j.t_begin = tbk;
-            tbk += j.tb;
             j.flags = flags + fbytes;
-            fbytes += 2 * uint64_t(j.B) * j.TH * j.TW;
+            fbytes += uint64_t(j.B) * j.TH * j.TW;
         }
     }
     if (use_band && eb > 0)
     {
-        THULL_CHECK(eb < (uint64_t(1) << 31) && tbk < (uint64_t(1) << 31), "blur batch too large");
+        THULL_CHECK(eb < (uint64_t(1) << 31), "blur batch too large");
         THULL_HIP(hipMemsetAsync(flags, 0, fbytes, st));
         THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
         if (vec)
@@ -439,8 +460,6 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
         }
         else
             blur_edge_flags_kernel<Ti, RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
-        THULL_LAUNCH_CHECK();
-        blur_band_tiles_kernel<<<unsigned(tbk), 256, 0, st>>>(djobs, int32_t(jobs.size()));
         THULL_LAUNCH_CHECK();
         // the tile pass uses its own block layout; the host table must stay valid until the copy ran
         THULL_HIP(hipStreamSynchronize(st));

@@ -152,13 +152,28 @@ grid_to_world(uint32_t v, float inv_res, float length, float corner)
 }
 
 // ---------------------------------------------------------------------------
+// Padded integral-image layout used by carving: per camera a plane of H + 1
+// rows of pitch integral_pitch(W) floats; element (y, x) is at row y + 1,
+// column x + 4, so every row's data starts 16-byte aligned.  Row 0 and
+// column 3 hold zeros: they are the entries y = -1 and x = -1 that the box
+// sums read at the image border.  Columns 0..2 and those past x = W - 1 are
+// never read.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ int64_t
+integral_pitch(int64_t W)
+{
+    return ((W + 3) & ~int64_t(3)) + 4;
+}
+
+// ---------------------------------------------------------------------------
 // Scene descriptor (device side).  One per scene of a batch.
 // ---------------------------------------------------------------------------
 
 struct SceneDev
 {
     const void* masks;      // [B,H,W] contiguous, element type mask_dtype
-    const float* integral;  // [B,H,W] float32 integral image
+    const float* integral;  // B planes of the padded integral-image layout (integral_pitch)
     const float* transforms; // [B,16] float32
     int32_t mask_dtype;
     int32_t B, H, W;
@@ -174,6 +189,33 @@ struct SceneDev
     uint32_t cell_row_words;
     uint64_t cell_offset;   // in words, into the batch's cell bitmap
 };
+
+// ---------------------------------------------------------------------------
+// Global-memory pointers.  Pointers read from a table in device memory are
+// generic (flat) to the compiler; casting them to the global address space
+// gives global loads and stores, which carry no LDS dependency and take
+// scalar base addresses.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T>
+as_global(T* p)
+{
+    return (gptr<T>)p;
+}
+
+typedef float float4_v __attribute__((ext_vector_type(4)));
+
+// 16-byte load through a global pointer (p 16-byte aligned).
+__device__ __forceinline__ float4
+ld_float4(gptr<const float> p)
+{
+    const float4_v v = *(gptr<const float4_v>)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 // ---------------------------------------------------------------------------
 // Segment lookup: index of the last segment whose first element is <= v, for

@@ -381,14 +381,17 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         max_level = std::max(max_level, d.level);
         if (d.level >= 0)
         {
-            float* integral = pool.alloc<float>(uint64_t(sc.B) * (sc.H + 1) * (sc.W + 1));
+            float* integral = pool.alloc<float>(uint64_t(sc.B) * (sc.H + 1) * uint64_t(integral_pitch(sc.W)));
             ireq.push_back(IntegralRequest{ sc.masks, sc.mask_dtype, sc.B, sc.H, sc.W, integral });
             d.integral = integral;
         }
         sd[s] = d;
     }
     // integral images of every scene in one launch per pass
-    launch_integral_images_padded(ireq.data(), int32_t(ireq.size()),
+    uint64_t carry_floats = 0;
+    for (const auto& r : ireq)
+        carry_floats += integral_carry_floats(r.B, r.H, r.W);
+    launch_integral_images_padded(ireq.data(), int32_t(ireq.size()), pool.alloc<float>(std::max<uint64_t>(carry_floats, 1)),
                                   pool.alloc_bytes(integral_job_bytes(std::max<size_t>(ireq.size(), 1))), st);
     word_begin[n] = total_words;
     ctx.scenes_host = sd;
@@ -471,14 +474,22 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
     uint32_t* prefix = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
-    launch_dilate_cells(cells, bitmap, total_words, dev_word_begin, scenes, n, st);
-    uint32_t* wsums = pool.alloc<uint32_t>(scan_blocks(total_words) + 1);
-    exclusive_scan(LoadPopcount{ bitmap }, total_words, prefix, wsums, dev_total, st);
+    // dilation + tile popcount sums, scan of the sums, then word prefixes + vertex keys
+    const uint32_t wblocks = scan_blocks(total_words);
+    uint32_t* wsums = pool.alloc<uint32_t>(wblocks + 1);
+    launch_dilate_cells(cells, bitmap, total_words, dev_word_begin, scenes, n, wsums, st);
+    if (wblocks > 0)
+    {
+        scan_block_sums_kernel<><<<1, 1024, 0, st>>>(wsums, wblocks, dev_total);
+        THULL_LAUNCH_CHECK();
+    }
+    else
+        THULL_HIP(hipMemsetAsync(dev_total, 0, sizeof(unsigned long long), st));
     const uint64_t nverts = ctx.read_u64(dev_total, st);
     check_u32(nverts, "sparse field size");
     uint64_t* vertices = pool.alloc<uint64_t>(std::max<uint64_t>(nverts, 1));
     float* values = pool.alloc<float>(std::max<uint64_t>(nverts, 1));
-    launch_extract_vertices(bitmap, prefix, total_words, dev_word_begin, scenes, n, vertices, st);
+    launch_prefix_extract(bitmap, total_words, wsums, dev_word_begin, scenes, n, prefix, vertices, st);
 
     // ---- field (VH:827-888) ----
     launch_field(vertices, uint32_t(nverts), scenes, values, st);
@@ -1208,8 +1219,9 @@ thull_integral_image(thull_ctx* ctx, const void* masks, int32_t dtype, int64_t B
         THULL_CHECK(H * W < (int64_t(1) << 31), "H * W must be below 2^31");
         hipStream_t st = static_cast<hipStream_t>(stream);
         ctx->c.begin(st);
-        thull::launch_integral_image(masks, dtype, B, H, W, out, ctx->c.pool.alloc_bytes(thull::integral_job_bytes(1)),
-                                     st);
+        float* carry = ctx->c.pool.alloc<float>(std::max<uint64_t>(thull::integral_carry_floats(B, H, W), 1));
+        thull::launch_integral_image(masks, dtype, B, H, W, out, carry,
+                                     ctx->c.pool.alloc_bytes(thull::integral_job_bytes(1)), st);
     });
 }
 

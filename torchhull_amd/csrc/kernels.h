@@ -9,14 +9,16 @@ namespace thull
 // ---- integral image (vh_kernels.hip) ----
 // float32 integral image of masks [B,H,W] of element type `dtype`, equal to
 // torch::cumsum(.., 1, f32) followed by cumsum(.., 2, f32) on CUDA.
-// `job_buf` is device scratch of integral_job_bytes(number of tensors) bytes.
+// `job_buf` is device scratch of integral_job_bytes(number of tensors) bytes;
+// `carry` holds integral_carry_floats(B, H, W) floats per tensor.
 size_t integral_job_bytes(size_t njobs);
-void launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, void* job_buf,
-                           hipStream_t st);
-// The same integral images, for many mask tensors in one launch, in the layout
-// carving reads: per camera a (H+1) x (W+1) plane whose row 0 and column 0 are
-// zero, so I(y, x) sits at (y+1)*(W+1) + (x+1) and the zero padding of
-// image_utils.h:84-97 needs no branch.
+uint64_t integral_carry_floats(int64_t B, int64_t H, int64_t W);
+void launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, float* carry,
+                           void* job_buf, hipStream_t st);
+// The same integral images, for many mask tensors in one launch per pass, in
+// the padded layout carving reads (integral_pitch in common.h): the zero row
+// and column make the zero padding of image_utils.h:84-97 branch-free.
+// `carry` holds the sum over requests of integral_carry_floats.
 struct IntegralRequest
 {
     const void* masks;
@@ -24,7 +26,7 @@ struct IntegralRequest
     int64_t B, H, W;
     float* padded;
 };
-void launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_buf, hipStream_t st);
+void launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, hipStream_t st);
 
 // ---- octree carving (vh_kernels.hip) ----
 constexpr int kCarveThreads = 256;
@@ -50,14 +52,17 @@ void launch_mark_leaf_cells(const uint64_t* parents, uint32_t n, const uint8_t* 
 // Sets the root cell of every level-0 scene.
 void launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells, hipStream_t st);
 // Vertex bitmap (all words, no clearing needed) = leaf cells dilated by {0,1}^3.
+// Also writes the popcount sum of every scan tile of words to block_sums
+// (scan_blocks(nwords) entries), the first pass of the vertex numbering.
 void launch_dilate_cells(const uint32_t* cells, uint32_t* verts, uint64_t nwords, const uint64_t* scene_word_begin,
-                         const SceneDev* scenes, int32_t nscenes, hipStream_t st);
+                         const SceneDev* scenes, int32_t nscenes, uint32_t* block_sums, hipStream_t st);
 
-// Writes the packed key of every set bit of the bitmap at prefix[word] + rank.
-// `scene_word_begin` has nscenes + 1 entries (word offsets of each scene).
-void launch_extract_vertices(const uint32_t* bitmap, const uint32_t* prefix, uint64_t nwords,
-                             const uint64_t* scene_word_begin, const SceneDev* scenes, int32_t nscenes,
-                             uint64_t* vertices, hipStream_t st);
+// From the exclusive block offsets of those sums: prefix[word] (exclusive
+// popcount prefix) and the packed key of every set bit of the bitmap at
+// prefix[word] + rank.  `scene_word_begin` has nscenes + 1 entries.
+void launch_prefix_extract(const uint32_t* bitmap, uint64_t nwords, const uint32_t* block_offsets,
+                           const uint64_t* scene_word_begin, const SceneDev* scenes, int32_t nscenes,
+                           uint32_t* prefix, uint64_t* vertices, hipStream_t st);
 
 // Field value of every vertex: product over cameras of the bilinear mask sample.
 void launch_field(const uint64_t* vertices, uint32_t n, const SceneDev* scenes, float* values, hipStream_t st);

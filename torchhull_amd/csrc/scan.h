@@ -64,12 +64,22 @@ scan_reduce_kernel(L load, uint64_t n, uint32_t* block_sums)
     __shared__ uint32_t lds[kScanThreads / 64];
     const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
     uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
+    if (base + kScanItems <= n)
     {
-        uint64_t i = base + k;
-        if (i < n)
-            s += load(i);
+        // full run of items: unconditional consecutive reads the compiler merges into wide loads
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            s += load(base + k);
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            uint64_t i = base + k;
+            if (i < n)
+                s += load(i);
+        }
     }
     uint32_t total;
     block_exclusive_sum(s, lds, total);
@@ -128,22 +138,47 @@ scan_downsweep_kernel(L load, uint64_t n, const uint32_t* block_offsets, uint32_
     const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
+    const bool full = base + kScanItems <= n;
+    if (full)
     {
-        uint64_t i = base + k;
-        v[k] = (i < n) ? load(i) : 0u;
-        s += v[k];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            v[k] = load(base + k);
+            s += v[k];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            uint64_t i = base + k;
+            v[k] = (i < n) ? load(i) : 0u;
+            s += v[k];
+        }
     }
     uint32_t total;
     uint32_t run = block_offsets[blockIdx.x] + block_exclusive_sum(s, lds, total);
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
+    if (full)
     {
-        uint64_t i = base + k;
-        if (i < n)
-            out[i] = run;
-        run += v[k];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            out[base + k] = run;
+            run += v[k];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+        {
+            uint64_t i = base + k;
+            if (i < n)
+                out[i] = run;
+            run += v[k];
+        }
     }
 }
 

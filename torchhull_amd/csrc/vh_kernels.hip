@@ -35,17 +35,6 @@ as_float<__half>(__half v)
     return __half2float(v);
 }
 
-// Mask sample as float (masks of any element type are used as float32,
-// image_utils.h:178).  The two native element types are f32 and u8 (bool);
-// other types are converted to f32 by the caller.
-__device__ __forceinline__ float
-mask_at(const void* masks, int32_t dtype, int64_t idx)
-{
-    if (dtype == kF32)
-        return static_cast<const float*>(masks)[idx];
-    return float(static_cast<const uint8_t*>(masks)[idx]);
-}
-
 // ===========================================================================
 // Integral image (VH:76-90)
 //
@@ -53,220 +42,220 @@ mask_at(const void* masks, int32_t dtype, int64_t idx)
 // tensor_kernel_scan_outer_dim (ATen/native/cuda/ScanUtils.cuh): one
 // sequential float chain per line, acc = acc + x from acc = 0.  Both passes of
 // the reference are of that kind on [B,H,W,1], so the integral image is
-// reproduced exactly by a sequential column pass followed by a sequential row
-// pass.  The parallelism is across lines: B*W column chains, then B*H row
-// chains.
+// reproduced exactly by a sequential column chain per column followed by a
+// sequential row chain per row.
+//
+// Two kernels compute it with one read of the output's worth of writes:
+//  * the carry pass runs every column chain from the top and stores its value
+//    only at the last row of each strip of kStripRows rows (1/32 of the data);
+//  * the strip pass gives one workgroup a strip of kStripRows rows and the
+//    full width: it resumes each column chain from the stored carry (the same
+//    sequence of additions, hence the same bits), puts the column sums of a
+//    kChunk-column chunk into LDS, runs the strip's row chains across the
+//    chunk (one lane per row), and writes the chunk out coalesced.  The next
+//    chunk's mask reads are in flight during the row chains.
 // ===========================================================================
 
+constexpr int kStripRows = 32;
+constexpr int kChunk = 256;
+
 // One integral-image job: B cameras of one mask tensor.  Blocks of every job
-// are flattened into one launch; `block_begin` is the job's first block.
+// are flattened into one launch per pass; `*_begin` is the job's first block.
 struct IntegralJob
 {
-    const void* masks;   // [B,H,W]
-    float* out;          // element (b, y, x) at out + b * plane + y * ld + x
+    const void* masks;    // [B,H,W]
+    float* out;           // element (b, y, x) at out + b * plane + y * ld + x
+    float* carry;         // B * (strips - 1) * W column-chain values at strip ends
     int64_t plane;
     int32_t ld;
     int32_t dtype;
     int32_t B, H, W;
-    int32_t vec4;         // float32 masks, W % 4 == 0, 16-byte aligned: 4 columns per thread
-    uint32_t col_blocks;  // blocks per camera of the column pass
-    uint32_t row_blocks;  // blocks per camera of the row pass
-    uint64_t col_begin;   // first block of this job in the column pass
-    uint64_t row_begin;   // first block of this job in the row pass
+    int32_t strips;       // ceil(H / kStripRows)
+    int32_t vec4;         // float32 masks, W % 4 == 0, 16-byte aligned: 4 columns per thread in the carry pass
+    uint32_t col_blocks;  // carry-pass blocks per camera
+    uint64_t col_begin;   // first block of this job in the carry pass
+    uint64_t strip_begin; // first block of this job in the strip pass (one block per camera and strip)
+    uint64_t cam_begin;   // first block of this job in the border pass (one block per camera)
 };
 
 __device__ __forceinline__ int32_t
-find_job(const IntegralJob* jobs, int32_t n, uint64_t blk, bool rows)
+find_job(const IntegralJob* jobs, int32_t n, uint64_t blk, int pass)
 {
-    if (rows)
-        return find_segment_uniform([&](int32_t i) { return jobs[i].row_begin; }, n, blk);
-    return find_segment_uniform([&](int32_t i) { return jobs[i].col_begin; }, n, blk);
+    if (pass == 0)
+        return find_segment_uniform([&](int32_t i) { return jobs[i].col_begin; }, n, blk);
+    if (pass == 1)
+        return find_segment_uniform([&](int32_t i) { return jobs[i].strip_begin; }, n, blk);
+    return find_segment_uniform([&](int32_t i) { return jobs[i].cam_begin; }, n, blk);
+}
+
+// Column chains down to the last strip boundary; `src` and `carry` point at
+// the thread's column.
+template <typename T>
+__device__ __forceinline__ void
+carry_column(gptr<const T> src, gptr<float> carry, int32_t strips, int32_t W)
+{
+    float acc = 0.0f;
+    for (int32_t s = 0; s + 1 < strips; ++s)
+    {
+        float v[kStripRows];
+#pragma unroll
+        for (int k = 0; k < kStripRows; ++k)
+            v[k] = as_float(src[int64_t(s * kStripRows + k) * W]);
+#pragma unroll
+        for (int k = 0; k < kStripRows; ++k)
+            acc = acc + v[k];
+        carry[int64_t(s) * W] = acc;
+    }
+}
+
+__device__ __forceinline__ void
+carry_column4(gptr<const float> src, gptr<float> carry, int32_t strips, int32_t W)
+{
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    for (int32_t s = 0; s + 1 < strips; ++s)
+    {
+#pragma unroll
+        for (int h = 0; h < kStripRows; h += 8)
+        {
+            float4 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                v[k] = ld_float4(src + int64_t(s * kStripRows + h + k) * W);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+            {
+                a0 = a0 + v[k].x;
+                a1 = a1 + v[k].y;
+                a2 = a2 + v[k].z;
+                a3 = a3 + v[k].w;
+            }
+        }
+        const auto c = carry + int64_t(s) * W;
+        c[0] = a0;
+        c[1] = a1;
+        c[2] = a2;
+        c[3] = a3;
+    }
+}
+
+__global__ __launch_bounds__(256) void
+integral_carry_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
+{
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, 0)];
+    const uint64_t local = blk - j.col_begin;
+    const int32_t b = int32_t(local / j.col_blocks);
+    const int32_t t = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    const int32_t x = j.vec4 ? 4 * t : t;
+    if (x >= j.W)
+        return;
+    const int64_t in_off = int64_t(b) * j.H * j.W + x;
+    const auto carry = as_global(j.carry + int64_t(b) * (j.strips - 1) * j.W + x);
+    if (j.vec4)
+        carry_column4(as_global(static_cast<const float*>(j.masks) + in_off), carry, j.strips, j.W);
+    else if (j.dtype == kF32)
+        carry_column(as_global(static_cast<const float*>(j.masks) + in_off), carry, j.strips, j.W);
+    else
+        carry_column(as_global(static_cast<const uint8_t*>(j.masks) + in_off), carry, j.strips, j.W);
 }
 
 template <typename T>
 __device__ __forceinline__ void
-integral_cols(const T* __restrict__ src, float* __restrict__ dst, int32_t H, int32_t W, int32_t ld)
+integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripRows][kChunk + 1])
 {
-    float acc = 0.0f;
-    int32_t y = 0;
-    constexpr int U = 16;
-    for (; y + U <= H; y += U)
+    const int32_t t = threadIdx.x;
+    const int32_t H = j.H, W = j.W, ld = j.ld;
+    const int32_t r0 = s * kStripRows;
+    const int32_t rows = min(kStripRows, H - r0);
+    const auto src = as_global(static_cast<const T*>(j.masks) + int64_t(b) * H * W + int64_t(r0) * W);
+    const auto dst = as_global(j.out + b * j.plane + int64_t(r0) * ld);
+    const auto carry = as_global(j.carry + (int64_t(b) * (j.strips - 1) + (s - 1)) * W);
+    float pre[kStripRows];
+    float cin = 0.0f;
+    // mask values of chunk c0 for this thread's column, rows past the image read as 0 (never stored)
+    const auto load_chunk = [&](int32_t c0) {
+        const int32_t x = c0 + t;
+        const bool in = x < W;
+#pragma unroll
+        for (int r = 0; r < kStripRows; ++r)
+            pre[r] = (in && r < rows) ? as_float(src[int64_t(r) * W + x]) : 0.0f;
+        cin = (in && s > 0) ? carry[x] : 0.0f;
+    };
+    load_chunk(0);
+    float racc = 0.0f; // row chain of row t (t < rows)
+    for (int32_t c0 = 0; c0 < W; c0 += kChunk)
     {
-        float v[U];
+        // column chains resumed from the carry: acc = acc + x, as in the carry pass
+        float acc = cin;
 #pragma unroll
-        for (int k = 0; k < U; ++k)
-            v[k] = as_float(src[int64_t(y + k) * W]);
-#pragma unroll
-        for (int k = 0; k < U; ++k)
+        for (int r = 0; r < kStripRows; ++r)
         {
-            acc = acc + v[k];
-            dst[int64_t(y + k) * ld] = acc;
+            acc = acc + pre[r];
+            tile[r][t] = acc;
         }
-    }
-    for (; y < H; ++y)
-    {
-        acc = acc + as_float(src[int64_t(y) * W]);
-        dst[int64_t(y) * ld] = acc;
-    }
-}
-
-// Column pass of 4 adjacent float32 columns per thread: 16-byte loads, four
-// independent chains.
-__device__ __forceinline__ void
-integral_cols4(const float* __restrict__ src, float* __restrict__ dst, int32_t H, int32_t W, int32_t ld)
-{
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    int32_t y = 0;
-    constexpr int U = 8;
-    for (; y + U <= H; y += U)
-    {
-        float4 v[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k)
-            v[k] = *reinterpret_cast<const float4*>(src + int64_t(y + k) * W);
-#pragma unroll
-        for (int k = 0; k < U; ++k)
-        {
-            a0 = a0 + v[k].x;
-            a1 = a1 + v[k].y;
-            a2 = a2 + v[k].z;
-            a3 = a3 + v[k].w;
-            float* d = dst + int64_t(y + k) * ld;
-            d[0] = a0;
-            d[1] = a1;
-            d[2] = a2;
-            d[3] = a3;
-        }
-    }
-    for (; y < H; ++y)
-    {
-        const float4 v = *reinterpret_cast<const float4*>(src + int64_t(y) * W);
-        a0 = a0 + v.x;
-        a1 = a1 + v.y;
-        a2 = a2 + v.z;
-        a3 = a3 + v.w;
-        float* d = dst + int64_t(y) * ld;
-        d[0] = a0;
-        d[1] = a1;
-        d[2] = a2;
-        d[3] = a3;
-    }
-}
-
-// Column pass: one thread per column (or per 4 columns), coalesced across the
-// wave, sequential over rows.
-__global__ __launch_bounds__(256) void
-integral_cols_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
-{
-    const uint64_t blk = blockIdx.x;
-    const IntegralJob j = jobs[find_job(jobs, njobs, blk, false)];
-    const uint64_t local = blk - j.col_begin;
-    const int32_t b = int32_t(local / j.col_blocks);
-    const int32_t t = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
-    if (j.vec4)
-    {
-        const int32_t x = 4 * t;
-        if (x >= j.W)
-            return;
-        integral_cols4(static_cast<const float*>(j.masks) + int64_t(b) * j.H * j.W + x, j.out + b * j.plane + x, j.H,
-                       j.W, j.ld);
-        return;
-    }
-    const int32_t x = t;
-    if (x >= j.W)
-        return;
-    const int64_t in_off = int64_t(b) * j.H * j.W + x;
-    float* dst = j.out + b * j.plane + x;
-    if (j.dtype == kF32)
-        integral_cols(static_cast<const float*>(j.masks) + in_off, dst, j.H, j.W, j.ld);
-    else
-        integral_cols(static_cast<const uint8_t*>(j.masks) + in_off, dst, j.H, j.W, j.ld);
-}
-
-// Row pass in place: one wavefront owns 64 rows and walks 64 x 32 tiles
-// through LDS, so loads and stores stay coalesced (two 128-byte row segments
-// per instruction) while every lane runs the sequential chain of its own row.
-// The next tile is loaded into registers while the current one is scanned.
-constexpr int kRowTileW = 32;
-
-__global__ __launch_bounds__(64) void
-integral_rows_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
-{
-    __shared__ float tile[64][kRowTileW + 1];
-    const uint64_t blk = blockIdx.x;
-    const IntegralJob j = jobs[find_job(jobs, njobs, blk, true)];
-    const uint64_t local = blk - j.row_begin;
-    const int32_t b = int32_t(local / j.row_blocks);
-    const int32_t r0 = int32_t(local - uint64_t(b) * j.row_blocks) * 64;
-    const int lane = threadIdx.x;
-    const int32_t rows = min(64, j.H - r0);
-    const int32_t W = j.W, ld = j.ld;
-    // element e of a lane's tile share: row 2e + lane / 32, column lane % 32
-    const int32_t lr = lane >> 5, lc = lane & 31;
-    float* base = j.out + b * j.plane + int64_t(r0 + lr) * ld + lc;
-    const int32_t nl = (rows - lr + 1) >> 1; // elements of this lane inside the strip
-    float pre[32];
-#pragma unroll
-    for (int e = 0; e < 32; ++e)
-        pre[e] = (e < nl && lc < W) ? base[int64_t(2 * e) * ld] : 0.0f;
-    float acc = 0.0f;
-    for (int32_t c0 = 0; c0 < W; c0 += kRowTileW)
-    {
-        const int32_t cols = min(kRowTileW, W - c0);
-#pragma unroll
-        for (int e = 0; e < 32; ++e)
-            tile[2 * e + lr][lc] = pre[e];
         __syncthreads();
-        const int32_t c1 = c0 + kRowTileW;
-        if (c1 < W)
+        if (c0 + kChunk < W)
+            load_chunk(c0 + kChunk);
+        if (t < rows)
         {
-            const bool cin = c1 + lc < W;
-#pragma unroll
-            for (int e = 0; e < 32; ++e)
-                pre[e] = (e < nl && cin) ? base[int64_t(2 * e) * ld + c1] : 0.0f;
-        }
-        if (lane < rows)
-        {
-            if (cols == kRowTileW)
+            const int32_t cols = min(kChunk, W - c0);
+            int32_t c = 0;
+            for (; c + 8 <= cols; c += 8)
             {
+                float v[8];
 #pragma unroll
-                for (int c = 0; c < kRowTileW; ++c)
+                for (int k = 0; k < 8; ++k)
+                    v[k] = tile[t][c + k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
                 {
-                    acc = acc + tile[lane][c];
-                    tile[lane][c] = acc;
+                    racc = racc + v[k];
+                    tile[t][c + k] = racc;
                 }
             }
-            else
-                for (int c = 0; c < cols; ++c)
-                {
-                    acc = acc + tile[lane][c];
-                    tile[lane][c] = acc;
-                }
+            for (; c < cols; ++c)
+            {
+                racc = racc + tile[t][c];
+                tile[t][c] = racc;
+            }
         }
         __syncthreads();
-        if (lc < cols)
+        if (c0 + t < W)
         {
 #pragma unroll
-            for (int e = 0; e < 32; ++e)
-                if (e < nl)
-                    base[int64_t(2 * e) * ld + c0] = tile[2 * e + lr][lc];
+            for (int r = 0; r < kStripRows; ++r)
+                if (r < rows)
+                    dst[int64_t(r) * ld + c0 + t] = tile[r][t];
         }
         __syncthreads();
     }
 }
 
-// Zero row and zero column of the padded layout used by carving (one block per camera and 256 entries).
+__global__ __launch_bounds__(256) void
+integral_strip_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
+{
+    __shared__ float tile[kStripRows][kChunk + 1];
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, 1)];
+    const uint64_t local = blk - j.strip_begin;
+    const int32_t b = int32_t(local / uint32_t(j.strips));
+    const int32_t s = int32_t(local - uint64_t(b) * uint32_t(j.strips));
+    if (j.dtype == kF32)
+        integral_strip<float>(j, b, s, tile);
+    else
+        integral_strip<uint8_t>(j, b, s, tile);
+}
+
+// Zero row and zero column of the padded layout used by carving (one block per camera).
 __global__ __launch_bounds__(256) void
 integral_border_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
 {
     const uint64_t blk = blockIdx.x;
-    const IntegralJob j = jobs[find_job(jobs, njobs, blk, false)];
-    const uint64_t local = blk - j.col_begin;
-    const int32_t b = int32_t(local / j.col_blocks);
-    const int32_t i = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, 2)];
+    const int32_t b = int32_t(blk - j.cam_begin);
     // padded plane base: out points at element (0, 0), one row and one column into the plane
-    float* p = j.out - j.ld - 1 + b * j.plane;
-    for (int32_t q = i; q <= max(j.H, j.W); q += int32_t(j.col_blocks) * 256)
+    const auto p = as_global(j.out - j.ld - 1 + b * j.plane);
+    for (int32_t q = threadIdx.x; q <= max(j.H, j.W); q += 256)
     {
         if (q <= j.W)
             p[q] = 0.0f;
@@ -275,36 +264,29 @@ integral_border_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     }
 }
 
-static void
-integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, bool border, hipStream_t st)
+static IntegralJob
+make_job(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, int64_t plane, int64_t ld)
 {
-    uint64_t cols = 0, rows = 0;
-    for (auto& j : jobs)
-    {
-        j.vec4 = j.dtype == kF32 && j.W % 4 == 0 && reinterpret_cast<uintptr_t>(j.masks) % 16 == 0;
-        j.col_blocks = blocks_for(j.vec4 ? j.W / 4 : j.W, 256);
-        j.row_blocks = blocks_for(j.H, 64);
-        j.col_begin = cols;
-        j.row_begin = rows;
-        cols += uint64_t(j.col_blocks) * j.B;
-        rows += uint64_t(j.row_blocks) * j.B;
-    }
-    if (cols == 0)
-        return;
-    THULL_CHECK(cols < (uint64_t(1) << 31) && rows < (uint64_t(1) << 31), "integral image batch too large");
-    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(IntegralJob), hipMemcpyHostToDevice, st));
-    const int32_t n = int32_t(jobs.size());
-    if (border)
-    {
-        integral_border_kernel<<<unsigned(cols), 256, 0, st>>>(djobs, n);
-        THULL_LAUNCH_CHECK();
-    }
-    integral_cols_kernel<<<unsigned(cols), 256, 0, st>>>(djobs, n);
-    THULL_LAUNCH_CHECK();
-    integral_rows_kernel<<<unsigned(rows), 64, 0, st>>>(djobs, n);
-    THULL_LAUNCH_CHECK();
-    // the job table is pageable host memory read by an async copy
-    THULL_HIP(hipStreamSynchronize(st));
+    IntegralJob j{};
+    j.masks = masks;
+    j.out = out;
+    j.plane = plane;
+    j.ld = int32_t(ld);
+    j.dtype = dtype;
+    j.B = int32_t(B);
+    j.H = int32_t(H);
+    j.W = int32_t(W);
+    j.strips = int32_t((H + kStripRows - 1) / kStripRows);
+    j.vec4 = dtype == kF32 && W % 4 == 0 && reinterpret_cast<uintptr_t>(masks) % 16 == 0;
+    return j;
+}
+
+uint64_t
+integral_carry_floats(int64_t B, int64_t H, int64_t W)
+{
+    return uint64_t(B) * uint64_t((H + kStripRows - 1) / kStripRows - 1 > 0 ? (H + kStripRows - 1) / kStripRows - 1
+                                                                               : 0) *
+           uint64_t(W);
 }
 
 size_t
@@ -313,20 +295,58 @@ integral_job_bytes(size_t njobs)
     return njobs * sizeof(IntegralJob);
 }
 
+static void
+integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* carry, bool border, hipStream_t st)
+{
+    uint64_t cols = 0, strips = 0, cams = 0, coff = 0;
+    for (auto& j : jobs)
+    {
+        j.col_blocks = blocks_for(j.vec4 ? j.W / 4 : j.W, 256);
+        j.col_begin = cols;
+        j.strip_begin = strips;
+        j.cam_begin = cams;
+        j.carry = carry + coff;
+        coff += integral_carry_floats(j.B, j.H, j.W);
+        if (j.strips > 1)
+            cols += uint64_t(j.col_blocks) * j.B;
+        strips += uint64_t(j.strips) * j.B;
+        cams += uint64_t(j.B);
+    }
+    if (strips == 0)
+        return;
+    THULL_CHECK(cols < (uint64_t(1) << 31) && strips < (uint64_t(1) << 31), "integral image batch too large");
+    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(IntegralJob), hipMemcpyHostToDevice, st));
+    const int32_t n = int32_t(jobs.size());
+    if (border)
+    {
+        integral_border_kernel<<<unsigned(cams), 256, 0, st>>>(djobs, n);
+        THULL_LAUNCH_CHECK();
+    }
+    if (cols > 0)
+    {
+        // jobs of a single strip have no carry-pass blocks and are never found by it
+        integral_carry_kernel<<<unsigned(cols), 256, 0, st>>>(djobs, n);
+        THULL_LAUNCH_CHECK();
+    }
+    integral_strip_kernel<<<unsigned(strips), 256, 0, st>>>(djobs, n);
+    THULL_LAUNCH_CHECK();
+    // the job table is pageable host memory read by an async copy
+    THULL_HIP(hipStreamSynchronize(st));
+}
+
 void
-launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, void* job_buf,
-                      hipStream_t st)
+launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, float* carry,
+                      void* job_buf, hipStream_t st)
 {
     if (B == 0 || H == 0 || W == 0)
         return;
     THULL_CHECK(dtype == kF32 || dtype == kU8 || dtype == kBool, "integral image: unsupported mask element type");
-    std::vector<IntegralJob> jobs(1);
-    jobs[0] = IntegralJob{ masks, out, H * W, int32_t(W), dtype, int32_t(B), int32_t(H), int32_t(W), 0, 0, 0, 0, 0 };
-    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), false, st);
+    std::vector<IntegralJob> jobs(1, make_job(masks, dtype, B, H, W, out, H * W, W));
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, false, st);
 }
 
 void
-launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_buf, hipStream_t st)
+launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, hipStream_t st)
 {
     std::vector<IntegralJob> jobs;
     for (int32_t i = 0; i < n; ++i)
@@ -336,10 +356,10 @@ launch_integral_images_padded(const IntegralRequest* req, int32_t n, void* job_b
             continue;
         THULL_CHECK(r.dtype == kF32 || r.dtype == kU8 || r.dtype == kBool,
                     "integral image: unsupported mask element type");
-        jobs.push_back(IntegralJob{ r.masks, r.padded + (r.W + 1) + 1, (r.H + 1) * (r.W + 1), int32_t(r.W + 1),
-                                    r.dtype, int32_t(r.B), int32_t(r.H), int32_t(r.W), 0, 0, 0, 0, 0 });
+        const int64_t ld = integral_pitch(r.W);
+        jobs.push_back(make_job(r.masks, r.dtype, r.B, r.H, r.W, r.padded + ld + 4, (r.H + 1) * ld, ld));
     }
-    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), true, st);
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, true, st);
 }
 
 // ===========================================================================
@@ -372,9 +392,10 @@ __device__ __forceinline__ CarveCamera
 load_camera(const float* __restrict__ T, bool gl)
 {
     CarveCamera c;
-    const float4 r0 = *reinterpret_cast<const float4*>(T);
-    const float4 r1 = *reinterpret_cast<const float4*>(T + 4);
-    const float4 rd = *reinterpret_cast<const float4*>(T + (gl ? 12 : 8));
+    const auto G = as_global(T);
+    const float4 r0 = ld_float4(G);
+    const float4 r1 = ld_float4(G + 4);
+    const float4 rd = ld_float4(G + (gl ? 12 : 8));
     c.x0 = r0.x;
     c.x1 = r0.y;
     c.x2 = r0.z;
@@ -424,13 +445,24 @@ clamp_round(float v, float hi)
     return int32_t(round_nonneg(fminf(fmaxf(v, 0.0f), hi)));
 }
 
-// One child against one camera (VH:236-280 and VH:365-412).  I is the camera's
-// integral image in the zero-padded layout, pitch ld = W + 1.  Returns true
-// when the camera classifies the child as empty.
+// Pixel window of one child for one camera (VH:236-247 and VH:365-380) and
+// the four integral-image reads of its box sum (VH:249-257).  I is the
+// camera's integral image in the zero-padded layout (integral_pitch), so the
+// reads need no bounds tests.  The reads are only issued here; the caller
+// uses them after the windows of all children are in flight.
+struct ChildWindow
+{
+    float i00, i01, i10, i11; // integral reads
+    float area;               // (y1 - y0 + 1) * (x1 - x0 + 1), one rounding like float(int64 area)
+    bool same;                // partial masks: clamping kept the box area (int64 wrap-around like the reference)
+    bool wide;                // partial masks: y1 - y0 > 1 && x1 - x0 > 1
+};
+
+typedef __attribute__((address_space(1))) const float global_float;
+
 template <bool PARTIAL>
-__device__ __forceinline__ bool
-child_camera(const Box& bb, bool fully_inside, const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1,
-             uint32_t bit, uint32_t& refine, uint32_t& inside_any)
+__device__ __forceinline__ void
+child_window(const Box& bb, const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1, ChildWindow& cw)
 {
     int32_t x0, y0, x1, y1;
     bool same = true;
@@ -460,20 +492,30 @@ child_camera(const Box& bb, bool fully_inside, const float* __restrict__ I, uint
             const uint64_t fx = uint64_t(f2i64_sat(rx1)) - uint64_t(f2i64_sat(rx0)) + 1u;
             same = int64_t(fy * fx) == int64_t(y1 - y0 + 1) * int64_t(x1 - x0 + 1);
         }
+        cw.wide = y1 - y0 > 1 && x1 - x0 > 1;
     }
-    // VH:249-257: four reads of the integral image; zero padding by the zero row / column
+    cw.same = same;
+    global_float* G = (global_float*)I;
     const uint32_t r0 = uint32_t(y0) * ld, r1 = uint32_t(y1 + 1) * ld;
-    const float i11 = I[r1 + uint32_t(x1) + 1u];
-    const float i00 = I[r0 + uint32_t(x0)];
-    const float i10 = I[r1 + uint32_t(x0)];
-    const float i01 = I[r0 + uint32_t(x1) + 1u];
-    const float sum = ((i11 + i00) - i10) - i01;
-    const float area = float(y1 - y0 + 1) * float(x1 - x0 + 1); // == float(int64 area): one rounding of the product
+    cw.i11 = G[r1 + uint32_t(x1) + 1u];
+    cw.i00 = G[r0 + uint32_t(x0)];
+    cw.i10 = G[r1 + uint32_t(x0)];
+    cw.i01 = G[r0 + uint32_t(x1) + 1u];
+    cw.area = float(y1 - y0 + 1) * float(x1 - x0 + 1);
+}
+
+// Decision of one child for one camera from its window (VH:258-280 and
+// VH:381-412).  Returns true when the camera classifies the child as empty.
+template <bool PARTIAL>
+__device__ __forceinline__ bool
+child_decide(const ChildWindow& cw, bool fully_inside, uint32_t bit, uint32_t& refine, uint32_t& inside_any)
+{
+    const float sum = ((cw.i11 + cw.i00) - cw.i10) - cw.i01;
     if constexpr (!PARTIAL)
     {
         if (sum <= kMargin)
             return true;
-        if (!(sum >= area - kMargin))
+        if (!(sum >= cw.area - kMargin))
             refine |= bit;
         return false;
     }
@@ -481,11 +523,11 @@ child_camera(const Box& bb, bool fully_inside, const float* __restrict__ I, uint
     {
         if (fully_inside)
             inside_any |= bit;
-        if (sum <= kMargin && same)
+        if (sum <= kMargin && cw.same)
             return true;
-        if (sum >= area - kMargin && same)
+        if (sum >= cw.area - kMargin && cw.same)
             return false;
-        if (y1 - y0 > 1 && x1 - x0 > 1)
+        if (cw.wide)
             refine |= bit;
         return false;
     }
@@ -509,9 +551,10 @@ parent_camera(const SceneDev& s,
     const float Hm1 = float(s.H - 1), Wm1 = float(s.W - 1);
     // VH:358-362 with roundf folded into the comparison: 1 <= roundf(p) <= n-2  <=>  0.5 <= p < n-1.5
     const float Wlim = float(s.W) - 1.5f, Hlim = float(s.H) - 1.5f;
-    const uint32_t ld = uint32_t(s.W) + 1u;
+    const uint32_t ld = uint32_t(integral_pitch(s.W));
     const CarveCamera cam = load_camera(s.transforms + 16 * b, gl);
-    const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld;
+    // entry (y, x), y and x from -1, at I[(y + 1) * ld + x + 1]
+    const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld + 3;
 
     Box win[3][4];
     uint32_t inbits[3] = { 0u, 0u, 0u };
@@ -573,6 +616,17 @@ parent_camera(const SceneDev& s,
                 win[w][a + 2 * bb] = Box{ fminf(mnx0, mnx1), fminf(mny0, mny1), fmaxf(mxx0, mxx1), fmaxf(mxy0, mxy1) };
             }
     }
+    // all live children's integral reads are issued before the first is used
+    ChildWindow cw[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        if (!(alive & (1u << c)))
+            continue;
+        const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
+        const Box box = box_merge(win[cz][a + 2 * bb], win[cz + 1][a + 2 * bb]);
+        child_window<PARTIAL>(box, I, ld, Hm1, Wm1, cw[c]);
+    }
     uint32_t dead = 0u;
 #pragma unroll
     for (int c = 0; c < 8; ++c)
@@ -580,15 +634,14 @@ parent_camera(const SceneDev& s,
         const uint32_t bit = 1u << c;
         if (!(alive & bit))
             continue;
-        const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
-        const Box box = box_merge(win[cz][a + 2 * bb], win[cz + 1][a + 2 * bb]);
         bool inside = false;
         if constexpr (PARTIAL)
         {
+            const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
             const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * bb);
             inside = ((inbits[cz] & mm) == mm) && ((inbits[cz + 1] & mm) == mm);
         }
-        if (child_camera<PARTIAL>(box, inside, I, ld, Hm1, Wm1, bit, refine, inside_any))
+        if (child_decide<PARTIAL>(cw[c], inside, bit, refine, inside_any))
             dead |= bit;
     }
     return dead;
@@ -944,21 +997,9 @@ mark_root_cells_kernel(const SceneDev* __restrict__ scenes, int32_t nscenes, uin
 
 // Vertex bitmap = leaf cells dilated by {0,1}^3: vertex (x, y, z) is a corner
 // of cell (x - dx, y - dy, z - dz).  One thread per vertex word, no atomics.
-__global__ __launch_bounds__(256) void
-dilate_cells_kernel(const uint32_t* __restrict__ cells,
-                    uint32_t* __restrict__ verts,
-                    uint64_t nwords,
-                    const uint64_t* __restrict__ scene_word_begin,
-                    const SceneDev* __restrict__ scenes,
-                    int32_t nscenes)
+__device__ __forceinline__ uint32_t
+dilate_word(const SceneDev& s, uint64_t local, const uint32_t* __restrict__ cells)
 {
-    const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    // all lanes take part in the lookup ballot before any exits
-    const int32_t lo = find_segment_rising([&](int32_t i) { return scene_word_begin[i]; }, nscenes, w);
-    if (w >= nwords)
-        return;
-    const SceneDev& s = scenes[lo];
-    const uint64_t local = w - scene_word_begin[lo];
     const uint64_t R = uint64_t(1) << s.level, R1 = R + 1;
     const uint64_t row = div_u64_u32(local, s.row_words);
     const uint32_t xw = uint32_t(local - row * s.row_words);
@@ -984,7 +1025,52 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
             out |= cw | (cw << 1) | (pw >> 31);
         }
     }
-    verts[w] = out;
+    return out;
+}
+
+// Each block covers kScanTile consecutive words, each thread one word per
+// 256; the scene lookup is done once per wave and step.  The block also sums
+// the popcounts of its words: the first pass of the exclusive scan that
+// numbers the vertices (the tile is that of the scan's down-sweep).
+__global__ __launch_bounds__(kScanThreads) void
+dilate_cells_kernel(const uint32_t* __restrict__ cells,
+                    uint32_t* __restrict__ verts,
+                    uint64_t nwords,
+                    const uint64_t* __restrict__ scene_word_begin,
+                    const SceneDev* __restrict__ scenes,
+                    int32_t nscenes,
+                    uint32_t* __restrict__ block_sums)
+{
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t w0 = uint64_t(blockIdx.x) * kScanTile + threadIdx.x;
+    const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
+    // all lanes take part in the lookup ballot before any diverge
+    int32_t lo = find_segment_rising(begin, nscenes, w0);
+    uint32_t count = 0u;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        const uint64_t w = w0 + uint64_t(k) * kScanThreads;
+        while (lo + 1 < nscenes && begin(lo + 1) <= w)
+            ++lo;
+        if (w >= nwords)
+            break;
+        const uint64_t local = w - begin(lo);
+        const int32_t l0 = __builtin_amdgcn_readfirstlane(lo);
+        uint32_t out;
+        if (__all(lo == l0))
+        {
+            const SceneDev sd = scenes[l0]; // wave-uniform: scalar loads
+            out = dilate_word(sd, local, cells);
+        }
+        else
+            out = dilate_word(scenes[lo], local, cells);
+        verts[w] = out;
+        count += __popc(out);
+    }
+    uint32_t total;
+    block_exclusive_sum(count, lds, total);
+    if (threadIdx.x == 0)
+        block_sums[blockIdx.x] = total;
 }
 
 void
@@ -1006,63 +1092,104 @@ launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells,
 
 void
 launch_dilate_cells(const uint32_t* cells, uint32_t* verts, uint64_t nwords, const uint64_t* scene_word_begin,
-                    const SceneDev* scenes, int32_t nscenes, hipStream_t st)
+                    const SceneDev* scenes, int32_t nscenes, uint32_t* block_sums, hipStream_t st)
 {
     if (nwords == 0)
         return;
-    dilate_cells_kernel<<<blocks_for(int64_t(nwords), 256), 256, 0, st>>>(cells, verts, nwords, scene_word_begin,
-                                                                         scenes, nscenes);
+    dilate_cells_kernel<<<scan_blocks(nwords), kScanThreads, 0, st>>>(cells, verts, nwords, scene_word_begin, scenes,
+                                                                      nscenes, block_sums);
     THULL_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(256) void
-extract_vertices_kernel(const uint32_t* __restrict__ bitmap,
-                        const uint32_t* __restrict__ prefix,
-                        uint64_t nwords,
-                        const uint64_t* __restrict__ scene_word_begin,
-                        const SceneDev* __restrict__ scenes,
-                        int32_t nscenes,
-                        uint64_t* __restrict__ vertices)
+// Second pass of the vertex numbering fused with the vertex extraction: each
+// thread takes kScanItems consecutive words, turns the block offsets of the
+// dilation pass into the exclusive prefix of every word (stored for the
+// marching-cubes rank lookups) and writes the packed key of every set bit at
+// prefix[word] + rank, i.e. in bitmap order = per scene (z, y, x) order.
+__global__ __launch_bounds__(kScanThreads) void
+prefix_extract_kernel(const uint32_t* __restrict__ bitmap,
+                      uint64_t nwords,
+                      const uint32_t* __restrict__ block_offsets,
+                      const uint64_t* __restrict__ scene_word_begin,
+                      const SceneDev* __restrict__ scenes,
+                      int32_t nscenes,
+                      uint32_t* __restrict__ prefix,
+                      uint64_t* __restrict__ vertices)
 {
-    const uint64_t w = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    // scene of this word: last s with scene_word_begin[s] <= w (all lanes take part in the ballot)
-    const int32_t lo = find_segment_rising([&](int32_t i) { return scene_word_begin[i]; }, nscenes, w);
-    if (w >= nwords)
-        return;
-    uint32_t word = bitmap[w];
-    if (word == 0u)
-        return;
-    const SceneDev& s = scenes[lo];
-    const uint64_t local = w - scene_word_begin[lo];
-    const uint64_t R1 = (uint64_t(1) << s.level) + 1;
-    const uint64_t row = div_u64_u32(local, s.row_words);
-    const uint32_t xw = uint32_t(local - row * s.row_words);
-    const uint64_t zq = div_u64_u32(row, uint32_t(R1));
-    const uint32_t y = uint32_t(row - zq * R1), z = uint32_t(zq);
-    uint32_t pos = prefix[w];
-    while (word != 0u)
+    __shared__ uint32_t lds[kScanThreads / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+    const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
+    int32_t lo = find_segment_rising(begin, nscenes, base);
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+    const bool full = base + kScanItems <= nwords;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
     {
-        const uint32_t b = __builtin_ctz(word);
-        vertices[pos++] = pack_key(uint32_t(lo), xw * 32u + b, y, z);
-        word &= word - 1u;
+        v[k] = (full || base + k < nwords) ? bitmap[base + k] : 0u;
+        sum += __popc(v[k]);
+    }
+    uint32_t total;
+    uint32_t run = block_offsets[blockIdx.x] + block_exclusive_sum(sum, lds, total);
+    // grid position of word `base` in its scene, then advanced word by word
+    uint32_t xw = 0, y = 0, z = 0, rw = 1, R1 = 1;
+    bool placed = false;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        const uint64_t w = base + k;
+        if (w >= nwords)
+            break;
+        if (!placed || (lo + 1 < nscenes && begin(lo + 1) <= w))
+        {
+            while (lo + 1 < nscenes && begin(lo + 1) <= w)
+                ++lo;
+            const SceneDev& sd = scenes[lo];
+            rw = sd.row_words;
+            R1 = (1u << sd.level) + 1u;
+            const uint64_t local = w - begin(lo);
+            const uint64_t row = div_u64_u32(local, rw);
+            xw = uint32_t(local - row * rw);
+            const uint64_t zq = div_u64_u32(row, R1);
+            y = uint32_t(row - zq * R1);
+            z = uint32_t(zq);
+            placed = true;
+        }
+        prefix[w] = run;
+        uint32_t word = v[k];
+        while (word != 0u)
+        {
+            const uint32_t bit = __builtin_ctz(word);
+            vertices[run++] = pack_key(uint32_t(lo), xw * 32u + bit, y, z);
+            word &= word - 1u;
+        }
+        if (++xw == rw)
+        {
+            xw = 0;
+            if (++y == R1)
+            {
+                y = 0;
+                ++z;
+            }
+        }
     }
 }
 
 void
-launch_extract_vertices(const uint32_t* bitmap,
-                        const uint32_t* prefix,
-                        uint64_t nwords,
-                        const uint64_t* scene_word_begin,
-                        const SceneDev* scenes,
-                        int32_t nscenes,
-                        uint64_t* vertices,
-                        hipStream_t st)
+launch_prefix_extract(const uint32_t* bitmap,
+                      uint64_t nwords,
+                      const uint32_t* block_offsets,
+                      const uint64_t* scene_word_begin,
+                      const SceneDev* scenes,
+                      int32_t nscenes,
+                      uint32_t* prefix,
+                      uint64_t* vertices,
+                      hipStream_t st)
 {
     if (nwords == 0)
         return;
-    extract_vertices_kernel<<<blocks_for(int64_t(nwords), 256), 256, 0, st>>>(bitmap, prefix, nwords,
-                                                                                scene_word_begin, scenes, nscenes,
-                                                                                vertices);
+    prefix_extract_kernel<<<scan_blocks(nwords), kScanThreads, 0, st>>>(bitmap, nwords, block_offsets,
+                                                                        scene_word_begin, scenes, nscenes, prefix,
+                                                                        vertices);
     THULL_LAUNCH_CHECK();
 }
 
@@ -1083,6 +1210,90 @@ int64_round_trip(float r)
     return (r != r) ? 0.0f : fminf(fmaxf(r, -9223372036854775808.0f), 9223372036854775808.0f);
 }
 
+// Bilinear sample of one camera's mask at the projection of a world point
+// (VH:470-530, IU:162-214): the four reads are issued by field_sample and
+// combined by field_factor, so that the reads of several cameras are in
+// flight together.  `skip`: a partial-mask camera that does not see the point
+// well (VH:513-518) leaves the product unchanged.
+template <typename T>
+struct FieldSample
+{
+    T v00, v10, v01, v11;
+    float ty, tx;
+    bool i00, i10, i01, i11;
+    bool skip;
+};
+
+template <typename T>
+__device__ __forceinline__ void
+field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSample<T>& f)
+{
+    const auto T4 = as_global(s.transforms + 16 * b);
+    const float fH = float(s.H), fW = float(s.W);
+    const float cx = fmaf(T4[0], wx, fmaf(T4[1], wy, fmaf(T4[2], wz, T4[3])));
+    const float cy = fmaf(T4[4], wx, fmaf(T4[5], wy, fmaf(T4[6], wz, T4[7])));
+    float px, py;
+    if (s.opengl != 0)
+    {
+        const float cw = fmaf(T4[12], wx, fmaf(T4[13], wy, fmaf(T4[14], wz, T4[15])));
+        px = unnormalize_ndc(cx / cw, fW);
+        py = unnormalize_ndc(cy / cw, fH);
+    }
+    else
+    {
+        const float cz = fmaf(T4[8], wx, fmaf(T4[9], wy, fmaf(T4[10], wz, T4[11])));
+        px = cx / cz - 0.5f;
+        py = cy / cz - 0.5f;
+    }
+    f.skip = false;
+    if (s.partial != 0)
+    {
+        // VH:513-518: only cameras whose rounded pixel lies inside with margin 1
+        const float rx = roundf(px), ry = roundf(py);
+        f.skip = !(rx >= 1.0f && rx <= float(s.W - 2) && ry >= 1.0f && ry <= float(s.H - 2));
+    }
+    // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
+    // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
+    // range tests are exact in float for integral values and H, W < 2^24.
+    const float fy = int64_round_trip(roundf(py + -0.5f));
+    const float fx = int64_round_trip(roundf(px + -0.5f));
+    const bool y0in = fy >= 0.0f && fy < fH, y1in = fy >= -1.0f && fy < fH - 1.0f;
+    const bool x0in = fx >= 0.0f && fx < fW, x1in = fx >= -1.0f && fx < fW - 1.0f;
+    f.i00 = y0in && x0in;
+    f.i10 = y1in && x0in;
+    f.i01 = y0in && x1in;
+    f.i11 = y1in && x1in;
+    // in-plane offsets (H * W < 2^31); a sample outside the image reads element 0 and is replaced by padding
+    const uint32_t W = uint32_t(s.W);
+    const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -1.0f), fH)));
+    const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -1.0f), fW)));
+    const uint32_t o00 = iy * W + ix;
+    typedef __attribute__((address_space(1))) const T global_t;
+    global_t* M = (global_t*)(static_cast<const T*>(s.masks) + int64_t(b) * s.H * s.W);
+    f.v00 = M[f.i00 ? o00 : 0u];
+    f.v10 = M[f.i10 ? o00 + W : 0u];
+    f.v01 = M[f.i01 ? o00 + 1u : 0u];
+    f.v11 = M[f.i11 ? o00 + W + 1u : 0u];
+    f.ty = py - fy;
+    f.tx = px - fx;
+}
+
+template <typename T>
+__device__ __forceinline__ float
+field_factor(const FieldSample<T>& f, float pad)
+{
+    const float v00 = f.i00 ? float(f.v00) : pad;
+    const float v10 = f.i10 ? float(f.v10) : pad;
+    const float v01 = f.i01 ? float(f.v01) : pad;
+    const float v11 = f.i11 ? float(f.v11) : pad;
+    const float l0 = lerp_ref(v00, v10, f.ty);
+    const float l1 = lerp_ref(v01, v11, f.ty);
+    return lerp_ref(l0, l1, f.tx);
+}
+
+// Product over the cameras, in camera order, of the bilinear mask samples at
+// a grid vertex (VH:470-530); masks of element type T (float32 or uint8).
+template <typename T>
 __device__ __forceinline__ float
 field_value(const SceneDev& s, uint64_t key)
 {
@@ -1090,60 +1301,27 @@ field_value(const SceneDev& s, uint64_t key)
     const float wx = grid_to_world(key_x(key), inv, s.length, s.corner[0]);
     const float wy = grid_to_world(key_y(key), inv, s.length, s.corner[1]);
     const float wz = grid_to_world(key_z(key), inv, s.length, s.corner[2]);
-    const bool gl = s.opengl != 0;
-    const bool partial = s.partial != 0;
-    const int64_t H = s.H, W = s.W;
-    const float fH = float(s.H), fW = float(s.W);
-    const float Hm2 = float(s.H - 2), Wm2 = float(s.W - 2);
-    const int64_t plane = H * W;
-    const float pad = partial ? 1.0f : 0.0f; // ones padding (partial) or zeros padding (full), IU:162-214
+    const float pad = s.partial != 0 ? 1.0f : 0.0f; // ones padding (partial) or zeros padding (full), IU:162-214
+    constexpr int U = 4; // cameras whose reads are in flight together
     float value = 1.0f;
-    for (int32_t b = 0; b < s.B; ++b)
+    int32_t b = 0;
+    for (; b + U <= s.B; b += U)
     {
-        const float* T = s.transforms + 16 * b;
-        const float cx = fmaf(T[0], wx, fmaf(T[1], wy, fmaf(T[2], wz, T[3])));
-        const float cy = fmaf(T[4], wx, fmaf(T[5], wy, fmaf(T[6], wz, T[7])));
-        float px, py;
-        if (gl)
-        {
-            const float cw = fmaf(T[12], wx, fmaf(T[13], wy, fmaf(T[14], wz, T[15])));
-            px = unnormalize_ndc(cx / cw, fW);
-            py = unnormalize_ndc(cy / cw, fH);
-        }
-        else
-        {
-            const float cz = fmaf(T[8], wx, fmaf(T[9], wy, fmaf(T[10], wz, T[11])));
-            px = cx / cz - 0.5f;
-            py = cy / cz - 0.5f;
-        }
-        if (partial)
-        {
-            // VH:513-518: only cameras whose rounded pixel lies inside with margin 1
-            const float rx = roundf(px), ry = roundf(py);
-            if (!(rx >= 1.0f && rx <= Wm2 && ry >= 1.0f && ry <= Hm2))
-                continue;
-        }
-        // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
-        // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
-        // range tests are exact in float for integral values and H, W < 2^24.
-        const float fy = int64_round_trip(roundf(py + -0.5f));
-        const float fx = int64_round_trip(roundf(px + -0.5f));
-        const bool y0in = fy >= 0.0f && fy < fH, y1in = fy >= -1.0f && fy < fH - 1.0f;
-        const bool x0in = fx >= 0.0f && fx < fW, x1in = fx >= -1.0f && fx < fW - 1.0f;
-        // in-plane offsets, exact whenever the sample is read (H * W < 2^31); wrapped otherwise
-        const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -1.0f), fH)));
-        const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -1.0f), fW)));
-        const uint32_t o00 = iy * uint32_t(W) + ix;
-        const int64_t base = b * plane;
-        const float v00 = (y0in && x0in) ? mask_at(s.masks, s.mask_dtype, base + o00) : pad;
-        const float v10 = (y1in && x0in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + uint32_t(W))) : pad;
-        const float v01 = (y0in && x1in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + 1u)) : pad;
-        const float v11 = (y1in && x1in) ? mask_at(s.masks, s.mask_dtype, base + (o00 + uint32_t(W) + 1u)) : pad;
-        const float ty = py - fy;
-        const float tx = px - fx;
-        const float l0 = lerp_ref(v00, v10, ty);
-        const float l1 = lerp_ref(v01, v11, ty);
-        value *= lerp_ref(l0, l1, tx);
+        FieldSample<T> f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            field_sample<T>(s, b + u, wx, wy, wz, f[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (!f[u].skip)
+                value *= field_factor<T>(f[u], pad);
+    }
+    for (; b < s.B; ++b)
+    {
+        FieldSample<T> f;
+        field_sample<T>(s, b, wx, wy, wz, f);
+        if (!f.skip)
+            value *= field_factor<T>(f, pad);
     }
     return value;
 }
@@ -1162,12 +1340,12 @@ field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* 
     if (__all(sid == s0))
     {
         const SceneDev s = scenes[s0]; // wave-uniform: scalar camera data
-        value = field_value(s, key);
+        value = s.mask_dtype == kF32 ? field_value<float>(s, key) : field_value<uint8_t>(s, key);
     }
     else
     {
         const SceneDev s = scenes[sid];
-        value = field_value(s, key);
+        value = s.mask_dtype == kF32 ? field_value<float>(s, key) : field_value<uint8_t>(s, key);
     }
     values[i] = value;
 }
