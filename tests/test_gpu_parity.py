@@ -260,7 +260,7 @@ def test_marching_cubes_empty():
 # ---------------------------------------------------------------------------
 
 
-@pytest.mark.parametrize("k", [1, 3, 7, 9, 15])
+@pytest.mark.parametrize("k", [1, 3, 5, 7, 9, 15, 31, 41])
 @pytest.mark.parametrize("out_dtype", [torch.float16, torch.float32, torch.float64])
 @pytest.mark.parametrize("in_dtype", [torch.uint8, torch.float32, torch.float64])
 def test_sparse_blur_matches_oracle(k, out_dtype, in_dtype):
@@ -272,6 +272,17 @@ def test_sparse_blur_matches_oracle(k, out_dtype, in_dtype):
     npd = {torch.float16: np.float16, torch.float32: np.float32, torch.float64: np.float64}[out_dtype]
     ref = oracle.gaussian_blur_sparse(mi.to(torch.float64).numpy(), k, 1.7, npd)
     assert got.dtype == ref.dtype
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 1, 70), (2, 70, 1), (1, 5, 3), (3, 65, 129)])
+@pytest.mark.parametrize("k", [3, 7, 9])
+def test_sparse_blur_small_shapes_match_oracle(shape, k):
+    th = _th()
+    rng = np.random.default_rng(7)
+    m = (rng.uniform(size=shape + (1,)) < 0.5).astype(np.float32)
+    got = th.gaussian_blur(torch.from_numpy(m).to(DEV), k, 1.3, sparse=True, dtype=torch.float32).cpu().numpy()
+    ref = oracle.gaussian_blur_sparse(m.astype(np.float64), k, 1.3, np.float32)
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
 
 

@@ -122,6 +122,12 @@ reflect_index(int64_t i, int64_t n)
         return i;
     if (n == 1)
         return 0;
+    // one reflection (the common case of a halo narrower than the image); at i = 1 - n the
+    // reference's formula below gives 0, not n - 1
+    if (i < 0 && i > 1 - n)
+        return -i;
+    if (i >= n && i < 2 * n - 1)
+        return 2 * n - 2 - i;
     const int64_t p = n - 1;
     const int64_t flips = i / p + (i < 0 ? 1 : 0);
     int64_t rem = i % p;
@@ -163,7 +169,7 @@ blur_edge_flags_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, int32_t 
 {
     constexpr int R = RB / 4;
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
+    const BlurJob j = jobs[find_blur_job(jobs, njobs, blk)]; // by value: no reloads after stores
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -204,7 +210,7 @@ blur_edge_flags_f32x4_kernel(const BlurJob* __restrict__ jobs, int32_t njobs, in
 {
     constexpr int R = RB / 4;
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
+    const BlurJob j = jobs[find_blur_job(jobs, njobs, blk)]; // by value: no reloads after stores
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(256) void
 blur_tile_kernel(const BlurJob* __restrict__ jobs,
                  int32_t njobs,
                  int32_t k_rt,
-                 int32_t ts,
+                 int32_t ts_rt,
                  bool sparse,
                  const To* __restrict__ weights,
                  To norm)
@@ -272,8 +278,9 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     To* tile = reinterpret_cast<To*>(smem_raw);
     constexpr int P = RB / 4; // pixels per thread
     const int32_t k = KT > 0 ? KT : k_rt;
+    const int32_t ts = KT > 0 ? (KT / 2 > 0 ? KT / 2 : 1) : ts_rt; // compile-time divisor when KT is fixed
     const uint64_t blk = blockIdx.x;
-    const BlurJob& j = jobs[find_blur_job(jobs, njobs, blk)];
+    const BlurJob j = jobs[find_blur_job(jobs, njobs, blk)]; // by value: no reloads after stores
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;
     const int32_t b = int32_t(local / per);
@@ -326,9 +333,10 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
             btl[q] = v;
         }
         __syncthreads();
+        const int32_t bx_t = x / ts - tx_lo;
 #pragma unroll
         for (int q = 0; q < P; ++q)
-            if (q < nq && btl[((y0 + ly0 + 4 * q) / ts - ty_lo) * bw + (x / ts - tx_lo)] != 0)
+            if (q < nq && btl[((y0 + ly0 + 4 * q) / ts - ty_lo) * bw + bx_t] != 0)
                 band |= 1u << q;
     }
     if (!__syncthreads_or(band != 0u))

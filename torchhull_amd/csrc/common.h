@@ -125,14 +125,14 @@ clamp_rounded(float r, float hi)
     return int32_t(fminf(fmaxf(r, 0.0f), hi));
 }
 
-// image_utils.h:67-71: ((c + 1) * size) / 2 - 0.5, every step rounded.
+// image_utils.h:67-71: ((c + 1) * size) / 2 - 0.5, every step rounded.  The
+// halving is exact, so fmaf(t, 0.5, -0.5) rounds once, like (t / 2) - 0.5.
 __device__ __forceinline__ float
 unnormalize_ndc(float c, float size)
 {
     float t = c + 1.0f;
     t = t * size;
-    t = t * 0.5f; // exact, equal to t / 2
-    return t - 0.5f;
+    return fmaf(t, 0.5f, -0.5f);
 }
 
 // math.h:19-23

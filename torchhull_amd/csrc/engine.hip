@@ -350,6 +350,9 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         THULL_CHECK(sc.cube_length > 0.0f, "cube_length must be positive");
         THULL_CHECK(sc.B >= 0 && sc.H >= 0 && sc.W >= 0, "mask sizes must be non-negative");
         THULL_CHECK(sc.H * sc.W < (int64_t(1) << 31), "H * W must be below 2^31");
+        // pixel rounding in float and 32-bit byte offsets into a camera's padded integral plane
+        THULL_CHECK(sc.H < (int64_t(1) << 23) && sc.W < (int64_t(1) << 23), "H and W must be below 2^23");
+        THULL_CHECK((sc.H + 1) * integral_pitch(sc.W) < (int64_t(1) << 30), "(H + 1) * (W + 4) must be below 2^30");
         THULL_CHECK(sc.mask_dtype == kF32 || sc.mask_dtype == kU8 || sc.mask_dtype == kBool,
                     "masks must be float32, uint8 or bool at the C ABI");
         SceneDev d{};
@@ -411,10 +414,15 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         roots[k] = pack_key(uint32_t(live[k]), 0, 0, 0);
     if (!roots.empty())
         THULL_HIP(hipMemcpyAsync(list, roots.data(), roots.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-    std::vector<uint8_t*> step_masks;
     uint32_t count = uint32_t(live.size());
     ctx.level_lists.push_back(list);
     ctx.level_sizes.push_back(count);
+
+    // leaf-cell bitmap, filled by the scatter of each scene's last carving step
+    uint32_t* cells = pool.alloc<uint32_t>(std::max<uint64_t>(total_cell_words, 1));
+    if (total_cell_words > 0)
+        THULL_HIP(hipMemsetAsync(cells, 0, total_cell_words * sizeof(uint32_t), st));
+    launch_mark_root_cells(scenes, n, cells, st);
 
     // ---- level loop (VH:634-751) ----
     for (int32_t i = 0; i < max_level && count > 0; ++i)
@@ -448,8 +456,10 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         }
         check_u32(next, "octree level size");
         uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(next, 1));
-        launch_carve_scatter(list, count, mask, boff, children, st);
-        step_masks.push_back(mask);
+        bool target = false; // a scene ends at level i + 1
+        for (int32_t s = 0; s < n; ++s)
+            target |= sd[s].level == i + 1;
+        launch_carve_scatter(list, count, mask, boff, children, i + 1, scenes, target ? cells : nullptr, st);
         list = children;
         count = uint32_t(next);
         ctx.level_lists.push_back(list);
@@ -457,19 +467,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     }
     const int32_t nlists = int32_t(ctx.level_lists.size());
 
-    // ---- corner vertices (VH:770-825): leaf cells, then their corners by dilation ----
-    uint32_t* cells = pool.alloc<uint32_t>(std::max<uint64_t>(total_cell_words, 1));
-    if (total_cell_words > 0)
-        THULL_HIP(hipMemsetAsync(cells, 0, total_cell_words * sizeof(uint32_t), st));
-    launch_mark_root_cells(scenes, n, cells, st);
-    for (int32_t i = 0; i + 1 < nlists; ++i)
-    {
-        bool target = false;
-        for (int32_t s = 0; s < n; ++s)
-            target |= sd[s].level == i + 1;
-        if (target)
-            launch_mark_leaf_cells(ctx.level_lists[i], ctx.level_sizes[i], step_masks[i], i + 1, scenes, cells, st);
-    }
+    // ---- corner vertices (VH:770-825): leaf cells (marked above), then their corners by dilation ----
     uint64_t* dev_word_begin = pool.alloc<uint64_t>(n + 1);
     THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));

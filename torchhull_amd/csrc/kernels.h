@@ -41,14 +41,12 @@ int32_t launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level
 int32_t carve_split_factor(uint32_t n);
 
 // Writes the occupied children of every parent, parent-major and child-minor,
-// at `block_offsets[block]` + the in-block prefix.
+// at `block_offsets[block]` + the in-block prefix.  With `cells`, the
+// children of parents whose scene ends at `child_level` are also set in the
+// leaf-cell bitmap.
 void launch_carve_scatter(const uint64_t* parents, uint32_t n, const uint8_t* child_mask,
-                          const uint32_t* block_offsets, uint64_t* children, hipStream_t st);
-
-// Sets the leaf-cell bits of the scenes whose target level is `level` from
-// the parents and child masks of carving step level - 1.
-void launch_mark_leaf_cells(const uint64_t* parents, uint32_t n, const uint8_t* child_mask, int32_t level,
-                            const SceneDev* scenes, uint32_t* cells, hipStream_t st);
+                          const uint32_t* block_offsets, uint64_t* children, int32_t child_level,
+                          const SceneDev* scenes, uint32_t* cells, hipStream_t st);
 // Sets the root cell of every level-0 scene.
 void launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells, hipStream_t st);
 // Vertex bitmap (all words, no clearing needed) = leaf cells dilated by {0,1}^3.

@@ -311,44 +311,63 @@ mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t
 // Pass 3: output vertices (compact mesh)
 // ---------------------------------------------------------------------------
 
+// The output vertices of a wavefront's 64 grid vertices are one contiguous
+// range (vbase is an exclusive prefix in vertex order): they are assembled in
+// LDS and written with coalesced stores.
+constexpr int kMcWaves = 256 / 64;
+
 template <typename Lookup, typename V>
 __global__ __launch_bounds__(256) void
 mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, const uint32_t* __restrict__ vbase,
                    float* __restrict__ verts)
 {
+    __shared__ float stage[kMcWaves][64 * 3 * 3];
     const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= p.nverts)
-        return;
-    const uint32_t m = emask[i];
-    if (m == 0u)
-        return;
-    const VertexRef r = vertex_ref(p, i);
-    const McGrid& g = p.grids[r.scene];
-    const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
-    const uint32_t p1[3] = { r.x, r.y, r.z };
-    // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
-    typename Lookup::Probe pr[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool valid = i < p.nverts;
+    const uint32_t m = valid ? emask[i] : 0u;
+    const uint32_t out = valid ? vbase[i] : 0u;
+    // wave range [V0, V1) of output vertices; lane 0 holds the smallest valid vertex
+    const uint32_t V0 = __builtin_amdgcn_readfirstlane(out);
+    uint32_t end = valid ? out + uint32_t(__popc(m & 7u)) : 0u;
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-        if (m & (1u << d))
-            pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
-    float v2[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-        if (m & (1u << d))
-            v2[d] = float(value_at<V>(p, lk.rank(pr[d])));
-    uint32_t out = vbase[i];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
+    for (int d = 32; d >= 1; d >>= 1)
+        end = max(end, __shfl_xor(end, d, 64));
+    if (m != 0u)
     {
-        if (!(m & (1u << d)))
-            continue;
-        const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
-        float v[3];
-        interpolate_vertex(p.iso, p1, p2, v1, v2[d], v);
-        finish_vertex(g, v, verts + 3 * uint64_t(out));
-        ++out;
+        const VertexRef r = vertex_ref(p, i);
+        const McGrid& g = p.grids[r.scene];
+        const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
+        const uint32_t p1[3] = { r.x, r.y, r.z };
+        // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
+        typename Lookup::Probe pr[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (m & (1u << d))
+                pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
+        float v2[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (m & (1u << d))
+                v2[d] = float(value_at<V>(p, lk.rank(pr[d])));
+        uint32_t o = out - V0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            if (!(m & (1u << d)))
+                continue;
+            const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
+            float v[3];
+            interpolate_vertex(p.iso, p1, p2, v1, v2[d], v);
+            finish_vertex(g, v, &stage[wave][3 * o]);
+            ++o;
+        }
     }
+    __syncthreads();
+    const uint32_t nf = 3u * (end - V0);
+    float* dst = verts + 3 * uint64_t(V0);
+    for (uint32_t q = uint32_t(lane); q < nf; q += 64)
+        dst[q] = stage[wave][q];
 }
 
 // ---------------------------------------------------------------------------
@@ -384,51 +403,6 @@ mc_faces_kernel(McPlan p,
     const uint32_t t0 = tbase[i];
     const uint32_t ts = scene_tstart[r.scene];
     const uint32_t vs = unique ? scene_vstart[r.scene] : 0u;
-    if (unique)
-    {
-        // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
-        // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
-        const uint32_t lows = c_cases.lows[ci];
-        typename Lookup::Probe pr[8];
-#pragma unroll
-        for (int o = 1; o < 8; ++o)
-            if (lows & (1u << o))
-                pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
-        uint32_t pos[8];
-        pos[0] = uint32_t(i);
-#pragma unroll
-        for (int o = 1; o < 8; ++o)
-            if (lows & (1u << o))
-                pos[o] = lk.rank(pr[o]);
-        uint32_t vb[8], em[8];
-#pragma unroll
-        for (int o = 0; o < 8; ++o)
-            if (lows & (1u << o))
-            {
-                vb[o] = vbase[pos[o]];
-                em[o] = emask[pos[o]];
-            }
-        for (int k = 0; k < count; ++k)
-        {
-            const int e = c_cases.edges[ci][k];
-            const uint32_t lo = c_edge_low[e];
-            const int axis = c_edge_axis[e];
-            uint32_t b = 0u, m = 0u;
-#pragma unroll
-            for (int o = 0; o < 8; ++o)
-                if (lo == uint32_t(o))
-                {
-                    b = vb[o];
-                    m = em[o];
-                }
-            const int64_t idx = int64_t(b + __popc(m & ((1u << axis) - 1u)) - vs);
-            int col = k % 3;
-            if (g.flip && col != 0)
-                col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
-            faces[3 * uint64_t((t0 + uint32_t(k)) / 3u) + col] = idx;
-        }
-        return;
-    }
     for (int k = 0; k < count; ++k)
     {
         const int e = c_cases.edges[ci][k];
@@ -457,6 +431,77 @@ mc_faces_kernel(McPlan p,
         if (g.flip && col != 0)
             col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
         faces[3 * uint64_t((t0 + uint32_t(k)) / 3u) + col] = idx;
+    }
+}
+
+// Faces of the compact mesh: per complete cell, the vertex index of each of
+// its triangle corners (an edge's rank in the edge numbering), written at the
+// cell's triangle-vertex slots (tbase, a multiple of 3) with flip_faces'
+// column swap.
+template <typename Lookup>
+__global__ __launch_bounds__(256) void
+mc_faces_unique_kernel(McPlan p,
+                       Lookup lk,
+                       const uint32_t* __restrict__ info,
+                       const uint32_t* __restrict__ tbase,
+                       const uint32_t* __restrict__ emask,
+                       const uint32_t* __restrict__ vbase,
+                       const uint32_t* __restrict__ scene_vstart,
+                       int64_t* __restrict__ faces)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= p.nverts)
+        return;
+    const uint32_t inf = info[i];
+    const uint32_t ci = inf & 0xffu;
+    const int count = (inf & kInfoComplete) ? c_cases.count[ci] : 0;
+    if (count > 0)
+    {
+        const uint32_t t0 = tbase[i];
+        const VertexRef r = vertex_ref(p, i);
+        const McGrid& g = p.grids[r.scene];
+        const uint32_t vs = scene_vstart[r.scene];
+        // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
+        // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
+        const uint32_t lows = c_cases.lows[ci];
+        typename Lookup::Probe pr[8];
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
+        uint32_t pos[8];
+        pos[0] = uint32_t(i);
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pos[o] = lk.rank(pr[o]);
+        uint32_t vb[8], em[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (lows & (1u << o))
+            {
+                vb[o] = vbase[pos[o]];
+                em[o] = emask[pos[o]];
+            }
+        int64_t* out = faces + uint64_t(t0);
+        for (int k = 0; k < count; ++k)
+        {
+            const int e = c_cases.edges[ci][k];
+            const uint32_t lo = c_edge_low[e];
+            const int axis = c_edge_axis[e];
+            uint32_t b = 0u, m = 0u;
+#pragma unroll
+            for (int o = 0; o < 8; ++o)
+                if (lo == uint32_t(o))
+                {
+                    b = vb[o];
+                    m = em[o];
+                }
+            int col = k % 3;
+            if (g.flip && col != 0)
+                col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
+            out[k - k % 3 + col] = int64_t(b + __popc(m & ((1u << axis) - 1u)) - vs);
+        }
     }
 }
 
@@ -542,8 +587,12 @@ launch_mc_faces(const McPlan& p,
     dispatch_mc(p, [&](auto lk, auto vt) {
         using LK = decltype(lk);
         using VT = decltype(vt);
-        mc_faces_kernel<LK, VT><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(
-                p, lk, info, ntv_base, emask, vbase, scene_vstart, scene_tstart, unique, soup_verts, faces);
+        if (unique)
+            mc_faces_unique_kernel<LK><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(
+                    p, lk, info, ntv_base, emask, vbase, scene_vstart, faces);
+        else
+            mc_faces_kernel<LK, VT><<<blocks_for(int64_t(p.nverts), 256), 256, 0, st>>>(
+                    p, lk, info, ntv_base, emask, vbase, scene_vstart, scene_tstart, unique, soup_verts, faces);
     });
     THULL_LAUNCH_CHECK();
 }

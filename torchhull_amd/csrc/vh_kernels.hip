@@ -428,21 +428,14 @@ box_merge(const Box& a, const Box& b)
                 fmaxf(a.maxy, b.maxy) };
 }
 
-// roundf (halves away from zero) of a value known to be >= 0.
-__device__ __forceinline__ float
-round_nonneg(float v)
-{
-    const float t = truncf(v);
-    return (v - t >= 0.5f) ? t + 1.0f : t;
-}
-
 // clamp(int64(roundf(v)), 0, hi) for the full-mask test: the clamp bounds are
 // integers and roundf is monotone, so clamping first gives the same integer,
-// NaN included (both map to 0).
+// NaN included (both map to 0).  For the clamped c in [0, hi], hi < 2^23,
+// c + 0.5 is exact and truncating it rounds halves up, which is roundf.
 __device__ __forceinline__ int32_t
 clamp_round(float v, float hi)
 {
-    return int32_t(round_nonneg(fminf(fmaxf(v, 0.0f), hi)));
+    return int32_t(fminf(fmaxf(v, 0.0f), hi) + 0.5f);
 }
 
 // Pixel window of one child for one camera (VH:236-247 and VH:365-380) and
@@ -495,12 +488,14 @@ child_window(const Box& bb, const float* __restrict__ I, uint32_t ld, float Hm1,
         cw.wide = y1 - y0 > 1 && x1 - x0 > 1;
     }
     cw.same = same;
-    global_float* G = (global_float*)I;
+    // 32-bit byte offsets from the camera's plane (a plane holds fewer than 2^30 floats), so the reads
+    // take the scalar-base + vector-offset form when the plane base is wave-uniform
+    const auto G = (gptr<const char>)I;
     const uint32_t r0 = uint32_t(y0) * ld, r1 = uint32_t(y1 + 1) * ld;
-    cw.i11 = G[r1 + uint32_t(x1) + 1u];
-    cw.i00 = G[r0 + uint32_t(x0)];
-    cw.i10 = G[r1 + uint32_t(x0)];
-    cw.i01 = G[r0 + uint32_t(x1) + 1u];
+    cw.i11 = *(global_float*)(G + (r1 + uint32_t(x1) + 1u) * 4u);
+    cw.i00 = *(global_float*)(G + (r0 + uint32_t(x0)) * 4u);
+    cw.i10 = *(global_float*)(G + (r1 + uint32_t(x0)) * 4u);
+    cw.i01 = *(global_float*)(G + (r0 + uint32_t(x1) + 1u) * 4u);
     cw.area = float(y1 - y0 + 1) * float(x1 - x0 + 1);
 }
 
@@ -844,12 +839,37 @@ carve_count_kernel(const uint8_t* __restrict__ child_mask, uint32_t n, uint32_t*
         block_counts[blockIdx.x] = total;
 }
 
+// The kept children of a parent whose scene ends at child_level are leaf
+// cells: set their bits in the scene's leaf-cell bitmap (rows of
+// cell_row_words words; the 2 x 2 x 2 children touch at most 4 words).
+__device__ __forceinline__ void
+mark_leaf_children(const SceneDev& sd, int32_t child_level, uint32_t cx, uint32_t cy, uint32_t cz, uint32_t m,
+                   uint32_t* __restrict__ cells)
+{
+    if (sd.level != child_level)
+        return;
+    const uint64_t R = uint64_t(1) << child_level;
+    uint32_t* base = cells + sd.cell_offset + (cx >> 5);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+        {
+            const uint32_t bits = (m >> (2 * b + 4 * c)) & 3u; // children (x = 0, 1; y = b; z = c)
+            if (bits != 0u)
+                atomicOr(base + (uint64_t(cz + c) * R + (cy + b)) * sd.cell_row_words, bits << (cx & 31u));
+        }
+}
+
 __global__ __launch_bounds__(kCarveThreads) void
 carve_scatter_kernel(const uint64_t* __restrict__ parents,
                      uint32_t n,
                      const uint8_t* __restrict__ child_mask,
                      const uint32_t* __restrict__ block_offsets,
-                     uint64_t* __restrict__ children)
+                     uint64_t* __restrict__ children,
+                     int32_t child_level,
+                     const SceneDev* __restrict__ scenes,
+                     uint32_t* __restrict__ cells)
 {
     __shared__ uint32_t lds[kCarveThreads / 64];
     const uint32_t t = blockIdx.x * kCarveThreads + threadIdx.x;
@@ -865,6 +885,8 @@ carve_scatter_kernel(const uint64_t* __restrict__ parents,
     for (int c = 0; c < 8; ++c)
         if (mask & (1u << c))
             children[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
+    if (cells != nullptr)
+        mark_leaf_children(scenes[s], child_level, x, y, z, mask, cells);
 }
 
 int32_t
@@ -914,12 +936,12 @@ launch_carve_classify(const uint64_t* parents,
 int32_t
 carve_split_factor(uint32_t n)
 {
-    // small levels split each parent's cameras over G lanes to fill the chip
+    // small levels split each parent's cameras over G lanes to fill the chip (about 10^6 lanes)
     if (n <= 16384)
         return 64;
-    if (n <= 65536)
+    if (n <= 131072)
         return 16;
-    if (n <= 262144)
+    if (n <= 1048576)
         return 4;
     return 1;
 }
@@ -930,12 +952,15 @@ launch_carve_scatter(const uint64_t* parents,
                      const uint8_t* child_mask,
                      const uint32_t* block_offsets,
                      uint64_t* children,
+                     int32_t child_level,
+                     const SceneDev* scenes,
+                     uint32_t* cells,
                      hipStream_t st)
 {
     if (n == 0)
         return;
-    carve_scatter_kernel<<<blocks_for(n, kCarveThreads), kCarveThreads, 0, st>>>(parents, n, child_mask,
-                                                                                block_offsets, children);
+    carve_scatter_kernel<<<blocks_for(n, kCarveThreads), kCarveThreads, 0, st>>>(
+            parents, n, child_mask, block_offsets, children, child_level, scenes, cells);
     THULL_LAUNCH_CHECK();
 }
 
@@ -954,38 +979,6 @@ launch_carve_scatter(const uint64_t* parents,
 // carving step level-1 kept, read from the parents' child masks.  A parent
 // owns a 2x2 block of bits in four cell rows, and both bits of a row sit in
 // one word (2 px is even), so it issues at most four atomics.
-__global__ __launch_bounds__(256) void
-mark_leaf_cells_kernel(const uint64_t* __restrict__ parents,
-                       uint32_t n,
-                       const uint8_t* __restrict__ child_mask,
-                       int32_t level,
-                       const SceneDev* __restrict__ scenes,
-                       uint32_t* __restrict__ cells)
-{
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n)
-        return;
-    const uint32_t m = child_mask[t];
-    if (m == 0u)
-        return;
-    const uint64_t key = parents[t];
-    const SceneDev& s = scenes[key_scene(key)];
-    if (s.level != level)
-        return;
-    const uint64_t R = uint64_t(1) << level;
-    const uint32_t cx = 2 * key_x(key), cy = 2 * key_y(key), cz = 2 * key_z(key);
-    uint32_t* base = cells + s.cell_offset + (cx >> 5);
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-        {
-            const uint32_t bits = (m >> (2 * b + 4 * c)) & 3u; // children (x=0,1, y=b, z=c)
-            if (bits != 0u)
-                atomicOr(base + (uint64_t(cz + c) * R + (cy + b)) * s.cell_row_words, bits << (cx & 31u));
-        }
-}
-
 // Level-0 scenes: the root cell is the only leaf.
 __global__ void
 mark_root_cells_kernel(const SceneDev* __restrict__ scenes, int32_t nscenes, uint32_t* __restrict__ cells)
@@ -1074,16 +1067,6 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
 }
 
 void
-launch_mark_leaf_cells(const uint64_t* parents, uint32_t n, const uint8_t* child_mask, int32_t level,
-                       const SceneDev* scenes, uint32_t* cells, hipStream_t st)
-{
-    if (n == 0)
-        return;
-    mark_leaf_cells_kernel<<<blocks_for(n, 256), 256, 0, st>>>(parents, n, child_mask, level, scenes, cells);
-    THULL_LAUNCH_CHECK();
-}
-
-void
 launch_mark_root_cells(const SceneDev* scenes, int32_t nscenes, uint32_t* cells, hipStream_t st)
 {
     mark_root_cells_kernel<<<blocks_for(nscenes, 256), 256, 0, st>>>(scenes, nscenes, cells);
@@ -1101,11 +1084,14 @@ launch_dilate_cells(const uint32_t* cells, uint32_t* verts, uint64_t nwords, con
     THULL_LAUNCH_CHECK();
 }
 
-// Second pass of the vertex numbering fused with the vertex extraction: each
-// thread takes kScanItems consecutive words, turns the block offsets of the
-// dilation pass into the exclusive prefix of every word (stored for the
-// marching-cubes rank lookups) and writes the packed key of every set bit at
-// prefix[word] + rank, i.e. in bitmap order = per scene (z, y, x) order.
+// Second pass of the vertex numbering fused with the vertex extraction.  Wave
+// w of a block takes the block's words [w * 1024, (w + 1) * 1024) in 16 steps
+// of 64 consecutive words (one per lane, coalesced); a wave prefix sum of
+// the popcounts plus a running total turn the block offsets of the dilation
+// pass into the exclusive prefix of every word (stored for the marching-cubes
+// rank lookups), and each set bit's packed key is written at prefix[word] +
+// rank, i.e. in bitmap order = per scene (z, y, x) order; adjacent lanes write
+// adjacent keys.
 __global__ __launch_bounds__(kScanThreads) void
 prefix_extract_kernel(const uint32_t* __restrict__ bitmap,
                       uint64_t nwords,
@@ -1116,60 +1102,61 @@ prefix_extract_kernel(const uint32_t* __restrict__ bitmap,
                       uint32_t* __restrict__ prefix,
                       uint64_t* __restrict__ vertices)
 {
-    __shared__ uint32_t lds[kScanThreads / 64];
-    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
-    const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
-    int32_t lo = find_segment_rising(begin, nscenes, base);
-    uint32_t v[kScanItems];
-    uint32_t sum = 0;
-    const bool full = base + kScanItems <= nwords;
+    constexpr int kWaveWords = kScanTile / (kScanThreads / 64);
+    constexpr int kSteps = kWaveWords / 64;
+    __shared__ uint32_t wave_total[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t wbase = uint64_t(blockIdx.x) * kScanTile + uint64_t(wave) * kWaveWords;
+    uint32_t v[kSteps];
+    uint32_t total = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
+    for (int k = 0; k < kSteps; ++k)
     {
-        v[k] = (full || base + k < nwords) ? bitmap[base + k] : 0u;
-        sum += __popc(v[k]);
+        const uint64_t w = wbase + 64 * k + lane;
+        v[k] = w < nwords ? bitmap[w] : 0u;
+        total += __popc(v[k]);
     }
-    uint32_t total;
-    uint32_t run = block_offsets[blockIdx.x] + block_exclusive_sum(sum, lds, total);
-    // grid position of word `base` in its scene, then advanced word by word
-    uint32_t xw = 0, y = 0, z = 0, rw = 1, R1 = 1;
-    bool placed = false;
-    for (int k = 0; k < kScanItems; ++k)
+    // wave total, then the offsets of the waves before this one
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        total += __shfl_xor(total, d, 64);
+    if (lane == 0)
+        wave_total[wave] = total;
+    __syncthreads();
+    uint32_t run = block_offsets[blockIdx.x];
+    for (int q = 0; q < wave; ++q)
+        run += wave_total[q];
+    const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
+    int32_t lo = find_segment_rising(begin, nscenes, wbase + lane);
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k)
     {
-        const uint64_t w = base + k;
+        const uint64_t w = wbase + 64 * k + lane;
+        const uint32_t c = __popc(v[k]);
+        const uint32_t inc = wave_inclusive_sum(c);
+        const uint32_t pos = run + inc - c;
+        run += __shfl(inc, 63, 64);
         if (w >= nwords)
-            break;
-        if (!placed || (lo + 1 < nscenes && begin(lo + 1) <= w))
-        {
-            while (lo + 1 < nscenes && begin(lo + 1) <= w)
-                ++lo;
-            const SceneDev& sd = scenes[lo];
-            rw = sd.row_words;
-            R1 = (1u << sd.level) + 1u;
-            const uint64_t local = w - begin(lo);
-            const uint64_t row = div_u64_u32(local, rw);
-            xw = uint32_t(local - row * rw);
-            const uint64_t zq = div_u64_u32(row, R1);
-            y = uint32_t(row - zq * R1);
-            z = uint32_t(zq);
-            placed = true;
-        }
-        prefix[w] = run;
+            continue;
+        prefix[w] = pos;
+        while (lo + 1 < nscenes && begin(lo + 1) <= w)
+            ++lo;
         uint32_t word = v[k];
+        if (word == 0u)
+            continue;
+        const SceneDev& sd = scenes[lo];
+        const uint32_t rw = sd.row_words, R1 = (1u << sd.level) + 1u;
+        const uint64_t local = w - begin(lo);
+        const uint64_t row = div_u64_u32(local, rw);
+        const uint32_t xw = uint32_t(local - row * rw);
+        const uint64_t zq = div_u64_u32(row, R1);
+        const uint32_t y = uint32_t(row - zq * R1), z = uint32_t(zq);
+        uint32_t o = pos;
         while (word != 0u)
         {
             const uint32_t bit = __builtin_ctz(word);
-            vertices[run++] = pack_key(uint32_t(lo), xw * 32u + bit, y, z);
+            vertices[o++] = pack_key(uint32_t(lo), xw * 32u + bit, y, z);
             word &= word - 1u;
-        }
-        if (++xw == rw)
-        {
-            xw = 0;
-            if (++y == R1)
-            {
-                y = 0;
-                ++z;
-            }
         }
     }
 }
@@ -1248,25 +1235,28 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
     f.skip = false;
     if (s.partial != 0)
     {
-        // VH:513-518: only cameras whose rounded pixel lies inside with margin 1
-        const float rx = roundf(px), ry = roundf(py);
-        f.skip = !(rx >= 1.0f && rx <= float(s.W - 2) && ry >= 1.0f && ry <= float(s.H - 2));
+        // VH:513-518: only cameras whose rounded pixel lies inside with margin 1;
+        // 1 <= roundf(p) <= n - 2  <=>  0.5 <= p < n - 1.5 (roundf rounds halves up for p >= 0)
+        f.skip = !(px >= 0.5f && px < float(s.W) - 1.5f && py >= 0.5f && py < float(s.H) - 1.5f);
     }
     // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
     // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
     // range tests are exact in float for integral values and H, W < 2^24.
     const float fy = int64_round_trip(roundf(py + -0.5f));
     const float fx = int64_round_trip(roundf(px + -0.5f));
-    const bool y0in = fy >= 0.0f && fy < fH, y1in = fy >= -1.0f && fy < fH - 1.0f;
-    const bool x0in = fx >= 0.0f && fx < fW, x1in = fx >= -1.0f && fx < fW - 1.0f;
+    // rows / columns as unsigned integers: clamping to [-2, 2^30] keeps every value outside [-1, n)
+    // outside, and -1 wraps to 2^32 - 1, so y0 in [0, H) <=> iy < H and y0 + 1 in [0, H) <=> iy + 1 < H
+    const uint32_t H = uint32_t(s.H), W = uint32_t(s.W);
+    const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -2.0f), 1073741824.0f)));
+    const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -2.0f), 1073741824.0f)));
+    const bool y0in = iy < H, y1in = iy + 1u < H;
+    const bool x0in = ix < W, x1in = ix + 1u < W;
     f.i00 = y0in && x0in;
     f.i10 = y1in && x0in;
     f.i01 = y0in && x1in;
     f.i11 = y1in && x1in;
-    // in-plane offsets (H * W < 2^31); a sample outside the image reads element 0 and is replaced by padding
-    const uint32_t W = uint32_t(s.W);
-    const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -1.0f), fH)));
-    const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -1.0f), fW)));
+    // in-plane offsets (H * W < 2^31), exact whenever the sample is read; a sample outside the image
+    // reads element 0 and is replaced by padding
     const uint32_t o00 = iy * W + ix;
     typedef __attribute__((address_space(1))) const T global_t;
     global_t* M = (global_t*)(static_cast<const T*>(s.masks) + int64_t(b) * s.H * s.W);
