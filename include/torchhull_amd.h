@@ -196,6 +196,13 @@ int thull_integral_image(thull_ctx* ctx,
                          float* out,
                          void* stream);
 
+/* Self-test with no counterpart in the reference: the carving and field kernels
+ * divide two numerators by one denominator with a shared reciprocal whenever the
+ * operands lie in [2^-40, 2^40] (the IEEE division otherwise).  This runs that
+ * helper and the IEEE division on `count` random operand triples on the device
+ * and returns the number of quotients whose bits differ (expected 0). */
+int thull_selftest_division(thull_ctx* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,8 @@ blur), vertices within 1e-6 * cube_length.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -303,3 +305,12 @@ def test_blur_dense_and_sparse_consistent(k, sigma):
     ref = torch.nn.functional.conv2d(x, w.T[None, None]).permute(0, 2, 3, 1)
     assert torch.allclose(dense.cpu().double(), ref, atol=1e-6)
     assert torch.all(dense >= -1e-5) and torch.all(dense <= 1 + 1e-5)
+
+
+def test_shared_reciprocal_division_equals_ieee_division():
+    # the carving and field kernels' two-quotient division against the IEEE division, 2^26 triples
+    lib = _lib()
+    n = ctypes.c_uint64(0)
+    lib.check(lib.lib().thull_selftest_division(lib.context(0), 1 << 26, 12345, ctypes.byref(n),
+                                                torch.cuda.current_stream().cuda_stream))
+    assert n.value == 0

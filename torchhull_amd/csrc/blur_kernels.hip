@@ -352,28 +352,42 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
                 st(O, int64_t(y0 + ly0 + 4 * q) * W + x, v[q]);
         return;
     }
-    // stage the block and its halo, converted to the output type (GB:221): wave w stages rows
-    // w, w + 4, ... in groups of 4 rows whose loads are issued together
-    for (int32_t cx = lx; cx < LW; cx += 64)
+    // stage the block and its halo, converted to the output type (GB:221): thread (lx, ly0) stages
+    // columns lx and lx + 64 of rows ly0, ly0 + 4, ...; reads go out in batches of 2 columns x 8 rows
+    // before the batch is stored
+    constexpr int kStageRows = 8;
     {
-        const int64_t xx = reflect_index(x0 + cx - h, W);
-        for (int32_t ty0 = ly0; ty0 < LH; ty0 += 16)
+        const int32_t cxa = lx, cxb = 64 + lx;
+        const bool inb = cxb < LW; // LW <= 128 when the halo fits in LDS with RB >= 16
+        const int64_t xa = reflect_index(x0 + cxa - h, W), xb = inb ? reflect_index(x0 + cxb - h, W) : 0;
+        for (int32_t t0 = ly0; t0 < LH; t0 += 4 * kStageRows)
         {
-            To v[4];
+            To va[kStageRows], vb[kStageRows];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kStageRows; ++u)
             {
-                const int32_t ty = ty0 + 4 * u;
-                if (ty < LH)
-                    v[u] = convert_in<To>(ld(I, reflect_index(y0 + ty - h, H) * W + xx));
+                const int32_t ty = t0 + 4 * u;
+                const int64_t row = reflect_index(y0 + min(ty, LH - 1) - h, H) * W;
+                va[u] = ty < LH ? convert_in<To>(ld(I, row + xa)) : OutOps<To>::zero();
+                vb[u] = (inb && ty < LH) ? convert_in<To>(ld(I, row + xb)) : OutOps<To>::zero();
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kStageRows; ++u)
             {
-                const int32_t ty = ty0 + 4 * u;
+                const int32_t ty = t0 + 4 * u;
                 if (ty < LH)
-                    tile[ty * LW + cx] = v[u];
+                {
+                    tile[ty * LW + cxa] = va[u];
+                    if (inb)
+                        tile[ty * LW + cxb] = vb[u];
+                }
             }
+        }
+        for (int32_t cx = 128 + lx; cx < LW; cx += 64) // halos wider than 32 pixels
+        {
+            const int64_t xx = reflect_index(x0 + cx - h, W);
+            for (int32_t ty = ly0; ty < LH; ty += 4)
+                tile[ty * LW + cx] = convert_in<To>(ld(I, reflect_index(y0 + ty - h, H) * W + xx));
         }
     }
     __syncthreads();

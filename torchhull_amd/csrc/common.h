@@ -142,6 +142,40 @@ lerp_ref(float v0, float v1, float t)
     return fmaf(t, v1, fmaf(-t, v0, v0));
 }
 
+// Two IEEE quotients nx / d and ny / d with one shared reciprocal.
+//
+// hipcc's correctly rounded float division is div_scale(n) and div_scale(d)
+// (operands scaled by 2^+-64 when exponents are extreme), r = rcp(d),
+// r += r * (1 - d * r), q = n * r, two corrections q += r * (n - d * q) (the
+// second through div_fmas, which undoes the scaling) and div_fixup (NaN,
+// infinities, zeros, overflow).  When 2^-40 <= |n|, |d| <= 2^40, div_scale
+// scales nothing (exponent gap < 96, no denormal operand, quotient or
+// reciprocal) and div_fixup passes the quotient through, so the same
+// operations without them give the same bits; the reciprocal does not depend
+// on the numerator, so both quotients share it.  Other operands take the
+// plain division.
+__device__ __forceinline__ void
+div2_shared(float nx, float ny, float d, float& qx, float& qy)
+{
+    const float hi = fmaxf(fmaxf(fabsf(nx), fabsf(ny)), fabsf(d));
+    const float lo = fminf(fminf(fabsf(nx), fabsf(ny)), fabsf(d));
+    if (hi <= 0x1p40f && lo >= 0x1p-40f)
+    {
+        float r = __builtin_amdgcn_rcpf(d);
+        r = fmaf(fmaf(-d, r, 1.0f), r, r);
+        float a = nx * r, b = ny * r;
+        a = fmaf(fmaf(-d, a, nx), r, a);
+        b = fmaf(fmaf(-d, b, ny), r, b);
+        qx = fmaf(fmaf(-d, a, nx), r, a);
+        qy = fmaf(fmaf(-d, b, ny), r, b);
+    }
+    else
+    {
+        qx = nx / d;
+        qy = ny / d;
+    }
+}
+
 // Grid coordinate -> world coordinate: corner + v / res * length, contracted
 // by nvcc to fmaf(v / res, length, corner) (visual_hull_cuda.cu:206-213).
 // res is a power of two, so v / res is exact and equal to v * inv_res.

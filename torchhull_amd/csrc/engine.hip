@@ -1223,4 +1223,18 @@ thull_integral_image(thull_ctx* ctx, const void* masks, int32_t dtype, int64_t B
     });
 }
 
+int
+thull_selftest_division(thull_ctx* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr && mismatches != nullptr, "ctx and mismatches must not be null");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        ctx->c.begin(st);
+        unsigned long long* dev = ctx->c.pool.alloc<unsigned long long>(1);
+        THULL_HIP(hipMemsetAsync(dev, 0, sizeof(unsigned long long), st));
+        thull::launch_div2_check(count, seed, dev, st);
+        *mismatches = ctx->c.read_u64(dev, st);
+    });
+}
+
 } // extern "C"

@@ -62,6 +62,10 @@ void launch_prefix_extract(const uint32_t* bitmap, uint64_t nwords, const uint32
                            const uint64_t* scene_word_begin, const SceneDev* scenes, int32_t nscenes,
                            uint32_t* prefix, uint64_t* vertices, hipStream_t st);
 
+// Self-test: div2_shared against the IEEE division on `count` random operand
+// triples; adds the number of differing quotients to *mismatches.
+void launch_div2_check(uint64_t count, uint64_t seed, unsigned long long* mismatches, hipStream_t st);
+
 // Field value of every vertex: product over cameras of the bilinear mask sample.
 void launch_field(const uint64_t* vertices, uint32_t n, const SceneDev* scenes, float* values, hipStream_t st);
 

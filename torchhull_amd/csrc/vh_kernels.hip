@@ -572,8 +572,8 @@ parent_camera(const SceneDev& s,
                 const float cx = fmaf(cam.x0, wx[u], bx);
                 const float cy = fmaf(cam.y0, wx[u], by);
                 const float cd = fmaf(cam.d0, wx[u], bd);
-                const float qx = cx / cd;
-                const float qy = cy / cd;
+                float qx, qy;
+                div2_shared(cx, cy, cd, qx, qy); // cx / cd, cy / cd
                 float pxl, pyl;
                 if (gl)
                 {
@@ -1181,6 +1181,60 @@ launch_prefix_extract(const uint32_t* bitmap,
 }
 
 // ===========================================================================
+// Self-test of div2_shared against the IEEE division on random operands in
+// (and at the edges of) its fast range and beyond it.
+// ===========================================================================
+
+__device__ __forceinline__ uint32_t
+mix32(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return uint32_t(x);
+}
+
+__device__ __forceinline__ float
+test_operand(uint64_t key)
+{
+    const uint32_t r = mix32(key), m = mix32(key ^ 0x9e3779b97f4a7c15ull);
+    // exponents -44..43 (the fast range is 2^-40 .. 2^40), mantissas random, all ones or zero
+    const int32_t e = int32_t(r % 88u) - 44;
+    const uint32_t mant = (m & 3u) == 0u ? 0x7fffffu : (m & 3u) == 1u ? 0u : (m >> 9);
+    return __uint_as_float(((r >> 31) << 31) | (uint32_t(e + 127) << 23) | mant);
+}
+
+__global__ __launch_bounds__(256) void
+div2_check_kernel(uint64_t count, uint64_t seed, unsigned long long* mismatches)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    uint32_t bad = 0u;
+    if (i < count)
+    {
+        const uint64_t k = seed * 0x100000001b3ull + 3 * i;
+        const float nx = test_operand(k), ny = test_operand(k + 1), d = test_operand(k + 2);
+        float qx, qy;
+        div2_shared(nx, ny, d, qx, qy);
+        const float rx = nx / d, ry = ny / d;
+        bad = (__float_as_uint(qx) != __float_as_uint(rx) ? 1u : 0u) + (__float_as_uint(qy) != __float_as_uint(ry) ? 1u : 0u);
+    }
+    bad = __reduce_add_sync(0xffffffffffffffffull, bad);
+    if ((threadIdx.x & 63) == 0 && bad != 0u)
+        atomicAdd(mismatches, (unsigned long long)bad);
+}
+
+void
+launch_div2_check(uint64_t count, uint64_t seed, unsigned long long* mismatches, hipStream_t st)
+{
+    if (count == 0)
+        return;
+    div2_check_kernel<<<blocks_for(int64_t(count), 256), 256, 0, st>>>(count, seed, mismatches);
+    THULL_LAUNCH_CHECK();
+}
+
+// ===========================================================================
 // Field (VH:419-520, camera loop VH:847)
 //
 // The reference launches one kernel per camera, each re-reading and
@@ -1223,14 +1277,18 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
     if (s.opengl != 0)
     {
         const float cw = fmaf(T4[12], wx, fmaf(T4[13], wy, fmaf(T4[14], wz, T4[15])));
-        px = unnormalize_ndc(cx / cw, fW);
-        py = unnormalize_ndc(cy / cw, fH);
+        float qx, qy;
+        div2_shared(cx, cy, cw, qx, qy); // cx / cw, cy / cw
+        px = unnormalize_ndc(qx, fW);
+        py = unnormalize_ndc(qy, fH);
     }
     else
     {
         const float cz = fmaf(T4[8], wx, fmaf(T4[9], wy, fmaf(T4[10], wz, T4[11])));
-        px = cx / cz - 0.5f;
-        py = cy / cz - 0.5f;
+        float qx, qy;
+        div2_shared(cx, cy, cz, qx, qy); // cx / cz, cy / cz
+        px = qx - 0.5f;
+        py = qy - 0.5f;
     }
     f.skip = false;
     if (s.partial != 0)
