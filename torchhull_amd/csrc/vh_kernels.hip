@@ -528,6 +528,107 @@ child_decide(const ChildWindow& cw, bool fully_inside, uint32_t bit, uint32_t& r
     }
 }
 
+// Projects one z-slice (3 x 3 points) of a parent's lattice for one camera and
+// reduces it to the pixel windows of its four 2 x 2 point groups, window a + 2b
+// for x offset a and y offset b; `inb` gets bit u + 3v for each point inside
+// the image with margin 1 (partial masks).  The fused row products are shared
+// along the lattice axes: fmaf(A2, z, A3) per slice, fmaf(A1, y, .) per row
+// and fmaf(A0, x, .) per point, the nested fmaf chain of VH:150-153.
+template <bool PARTIAL>
+__device__ __forceinline__ void
+lattice_slice(const CarveCamera& cam, bool gl, const float (&wx)[3], const float (&wy)[3], float wz, float fW, float fH,
+              float Wlim, float Hlim, Box (&win)[4], uint32_t& inb)
+{
+    const float ax = fmaf(cam.x2, wz, cam.x3);
+    const float ay = fmaf(cam.y2, wz, cam.y3);
+    const float ad = fmaf(cam.d2, wz, cam.d3);
+    float ppx[3][3], ppy[3][3]; // [v][u]
+    inb = 0u;
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+    {
+        const float bx = fmaf(cam.x1, wy[v], ax);
+        const float by = fmaf(cam.y1, wy[v], ay);
+        const float bd = fmaf(cam.d1, wy[v], ad);
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+        {
+            const float cx = fmaf(cam.x0, wx[u], bx);
+            const float cy = fmaf(cam.y0, wx[u], by);
+            const float cd = fmaf(cam.d0, wx[u], bd);
+            float qx, qy;
+            div2_shared(cx, cy, cd, qx, qy); // cx / cd, cy / cd
+            float pxl, pyl;
+            if (gl)
+            {
+                // image_utils.h:67-71; t * 0.5 is exact, so fmaf(t, 0.5, -0.5) == (t / 2) - 0.5
+                pxl = fmaf((qx + 1.0f) * fW, 0.5f, -0.5f);
+                pyl = fmaf((qy + 1.0f) * fH, 0.5f, -0.5f);
+            }
+            else
+            {
+                pxl = qx - 0.5f;
+                pyl = qy - 0.5f;
+            }
+            ppx[v][u] = pxl;
+            ppy[v][u] = pyl;
+            if constexpr (PARTIAL)
+            {
+                if (pxl >= 0.5f && pxl < Wlim && pyl >= 0.5f && pyl < Hlim)
+                    inb |= 1u << (u + 3 * v);
+            }
+        }
+    }
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+        {
+            const float mnx0 = fminf(ppx[bb][a], ppx[bb][a + 1]);
+            const float mnx1 = fminf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
+            const float mny0 = fminf(ppy[bb][a], ppy[bb][a + 1]);
+            const float mny1 = fminf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
+            const float mxx0 = fmaxf(ppx[bb][a], ppx[bb][a + 1]);
+            const float mxx1 = fmaxf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
+            const float mxy0 = fmaxf(ppy[bb][a], ppy[bb][a + 1]);
+            const float mxy1 = fmaxf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
+            win[a + 2 * bb] = Box{ fminf(mnx0, mnx1), fminf(mny0, mny1), fmaxf(mxx0, mxx1), fmaxf(mxy0, mxy1) };
+        }
+}
+
+// The four children of layer cz (children 4cz .. 4cz + 3) between slices cz
+// (`lo`) and cz + 1 (`hi`): their integral reads are issued before the first
+// is used.  Returns the children this camera finds empty.
+template <bool PARTIAL>
+__device__ __forceinline__ uint32_t
+child_layer(int cz, const Box (&lo)[4], const Box (&hi)[4], uint32_t ilo, uint32_t ihi, uint32_t alive,
+            const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1, uint32_t& refine, uint32_t& inside_any)
+{
+    ChildWindow cw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (alive & (1u << (4 * cz + q)))
+            child_window<PARTIAL>(box_merge(lo[q], hi[q]), I, ld, Hm1, Wm1, cw[q]);
+    uint32_t dead = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const uint32_t bit = 1u << (4 * cz + q);
+        if (!(alive & bit))
+            continue;
+        bool inside = false;
+        if constexpr (PARTIAL)
+        {
+            const int a = q & 1, bb = q >> 1;
+            const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * bb);
+            inside = ((ilo & mm) == mm) && ((ihi & mm) == mm);
+        }
+        if (child_decide<PARTIAL>(cw[q], inside, bit, refine, inside_any))
+            dead |= bit;
+    }
+    return dead;
+}
+
 // Projects the 3x3x3 lattice of one parent for one camera and classifies the
 // live children.  `dead` collects children found empty by this camera.
 template <bool PARTIAL>
@@ -551,94 +652,15 @@ parent_camera(const SceneDev& s,
     // entry (y, x), y and x from -1, at I[(y + 1) * ld + x + 1]
     const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld + 3;
 
-    Box win[3][4];
-    uint32_t inbits[3] = { 0u, 0u, 0u };
-#pragma unroll
-    for (int w = 0; w < 3; ++w)
-    {
-        const float ax = fmaf(cam.x2, wz[w], cam.x3);
-        const float ay = fmaf(cam.y2, wz[w], cam.y3);
-        const float ad = fmaf(cam.d2, wz[w], cam.d3);
-        float ppx[3][3], ppy[3][3]; // [v][u]
-#pragma unroll
-        for (int v = 0; v < 3; ++v)
-        {
-            const float bx = fmaf(cam.x1, wy[v], ax);
-            const float by = fmaf(cam.y1, wy[v], ay);
-            const float bd = fmaf(cam.d1, wy[v], ad);
-#pragma unroll
-            for (int u = 0; u < 3; ++u)
-            {
-                const float cx = fmaf(cam.x0, wx[u], bx);
-                const float cy = fmaf(cam.y0, wx[u], by);
-                const float cd = fmaf(cam.d0, wx[u], bd);
-                float qx, qy;
-                div2_shared(cx, cy, cd, qx, qy); // cx / cd, cy / cd
-                float pxl, pyl;
-                if (gl)
-                {
-                    // image_utils.h:67-71; t * 0.5 is exact, so fmaf(t, 0.5, -0.5) == (t / 2) - 0.5
-                    pxl = fmaf((qx + 1.0f) * fW, 0.5f, -0.5f);
-                    pyl = fmaf((qy + 1.0f) * fH, 0.5f, -0.5f);
-                }
-                else
-                {
-                    pxl = qx - 0.5f;
-                    pyl = qy - 0.5f;
-                }
-                ppx[v][u] = pxl;
-                ppy[v][u] = pyl;
-                if constexpr (PARTIAL)
-                {
-                    if (pxl >= 0.5f && pxl < Wlim && pyl >= 0.5f && pyl < Hlim)
-                        inbits[w] |= 1u << (u + 3 * v);
-                }
-            }
-        }
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-            {
-                const float mnx0 = fminf(ppx[bb][a], ppx[bb][a + 1]);
-                const float mnx1 = fminf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
-                const float mny0 = fminf(ppy[bb][a], ppy[bb][a + 1]);
-                const float mny1 = fminf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
-                const float mxx0 = fmaxf(ppx[bb][a], ppx[bb][a + 1]);
-                const float mxx1 = fmaxf(ppx[bb + 1][a], ppx[bb + 1][a + 1]);
-                const float mxy0 = fmaxf(ppy[bb][a], ppy[bb][a + 1]);
-                const float mxy1 = fmaxf(ppy[bb + 1][a], ppy[bb + 1][a + 1]);
-                win[w][a + 2 * bb] = Box{ fminf(mnx0, mnx1), fminf(mny0, mny1), fmaxf(mxx0, mxx1), fmaxf(mxy0, mxy1) };
-            }
-    }
-    // all live children's integral reads are issued before the first is used
-    ChildWindow cw[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        if (!(alive & (1u << c)))
-            continue;
-        const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
-        const Box box = box_merge(win[cz][a + 2 * bb], win[cz + 1][a + 2 * bb]);
-        child_window<PARTIAL>(box, I, ld, Hm1, Wm1, cw[c]);
-    }
-    uint32_t dead = 0u;
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        const uint32_t bit = 1u << c;
-        if (!(alive & bit))
-            continue;
-        bool inside = false;
-        if constexpr (PARTIAL)
-        {
-            const int a = c & 1, bb = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * bb);
-            inside = ((inbits[cz] & mm) == mm) && ((inbits[cz + 1] & mm) == mm);
-        }
-        if (child_decide<PARTIAL>(cw[c], inside, bit, refine, inside_any))
-            dead |= bit;
-    }
+    // one z-slice of the lattice at a time: the child boxes of layer cz need slices cz and cz + 1,
+    // so only two slices of windows are live (fewer registers, more waves in flight)
+    Box w0[4], w1[4];
+    uint32_t i0 = 0u, i1 = 0u;
+    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[0], fW, fH, Wlim, Hlim, w0, i0);
+    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[1], fW, fH, Wlim, Hlim, w1, i1);
+    uint32_t dead = child_layer<PARTIAL>(0, w0, w1, i0, i1, alive, I, ld, Hm1, Wm1, refine, inside_any);
+    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[2], fW, fH, Wlim, Hlim, w0, i0);
+    dead |= child_layer<PARTIAL>(1, w1, w0, i1, i0, alive, I, ld, Hm1, Wm1, refine, inside_any);
     return dead;
 }
 
