@@ -96,6 +96,9 @@ int thull_visual_hull_run(thull_ctx* ctx,
 
 /* Copies scene `scene`'s mesh: verts float32 [num_verts,3], faces int64 [num_faces,3]. */
 int thull_visual_hull_fetch_mesh(thull_ctx* ctx, int32_t scene, float* verts, int64_t* faces, void* stream);
+/* Copies every scene's mesh at once, scene after scene: verts float32 [sum num_verts,3],
+ * faces int64 [sum num_faces,3] (face indices relative to each scene's own vertices). */
+int thull_visual_hull_fetch_all(thull_ctx* ctx, float* verts, int64_t* faces, void* stream);
 
 /* Copies scene `scene`'s sparse field as COO: indices int64 [4,num_field] rows (0, z, y, x),
  * values float32 [num_field]; replaces sparse_visual_hull_field_cuda

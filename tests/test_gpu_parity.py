@@ -175,7 +175,8 @@ def test_blur_batch_equals_single():
         assert torch.equal(o.view(torch.int32), ref.view(torch.int32))
 
 
-def test_visual_hull_batch_equals_single():
+@pytest.mark.parametrize("unique", [True, False])
+def test_visual_hull_batch_equals_single(unique):
     th = _th()
     scenes = []
     for name in ["c0", "c0_partial", "c2_small", "c1_small"]:
@@ -184,9 +185,10 @@ def test_visual_hull_batch_equals_single():
         scenes.append(dict(masks=m, transforms=sc.transforms.to(DEV), level=sc.level,
                            cube_corner_bfl=sc.cube_corner_bfl, cube_length=sc.cube_length,
                            masks_partial=sc.masks_partial, transforms_convention=sc.transforms_convention))
-    batch = th.visual_hull_batch(scenes, unique_verts=True)
+    batch = th.visual_hull_batch(scenes, unique_verts=unique)
     for s, (vb, fb) in zip(scenes, batch):
-        v, f = th.visual_hull(**s, unique_verts=True)
+        v, f = th.visual_hull(**s, unique_verts=unique)
+        assert v.shape[0] > 0
         assert torch.equal(f, fb)
         assert torch.equal(v.view(torch.int32), vb.view(torch.int32))
 

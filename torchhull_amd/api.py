@@ -64,6 +64,8 @@ def _masks_for_abi(masks: torch.Tensor) -> tuple[torch.Tensor, int]:
     float32, uint8 and bool are read as they are; every other dtype is converted
     to float32 first, which is what the reference does per sample
     (image_utils.h:178 casts to float) and for the integral (cumsum in float32)."""
+    if masks.dtype in (torch.float32, torch.uint8, torch.bool) and masks.is_contiguous():
+        return masks, _DTYPE_CODES[masks.dtype]  # [B,H,W,1] contiguous has the [B,H,W] layout
     m = masks[..., 0]
     if m.dtype in (torch.float32, torch.uint8, torch.bool):
         m = m.contiguous()
@@ -92,7 +94,9 @@ def _scene_struct(masks: torch.Tensor, transforms: torch.Tensor, level: int, cub
     _check_vh_args(masks, transforms, level, cube_length)
     opengl = _convention(transforms_convention)
     m, code = _masks_for_abi(masks)
-    T = transforms.to(torch.float32).contiguous()  # fmaf arguments are float (visual_hull_cuda.cu:150-153)
+    T = transforms
+    if T.dtype != torch.float32 or not T.is_contiguous():
+        T = T.to(torch.float32).contiguous()  # fmaf arguments are float (visual_hull_cuda.cu:150-153)
     keep.extend([m, T])
     corner = [float(c) for c in cube_corner_bfl]
     _check(len(corner) == 3, "len(cube_corner_bfl) == 3")
@@ -121,6 +125,20 @@ class _Run:
         with torch.cuda.device(device):
             _lib.check(_lib.lib().thull_visual_hull_run(self.ctx, arr, len(scenes), int(bool(unique_verts)),
                                                         self.stream, self.counts))
+
+    def meshes(self) -> list[tuple[torch.Tensor, torch.Tensor]]:
+        """All scenes' meshes with two copies: views into one vertex and one face tensor."""
+        nv = [int(self.counts[i].num_verts) for i in range(len(self.counts))]
+        nf = [int(self.counts[i].num_faces) for i in range(len(self.counts))]
+        verts = torch.empty((sum(nv), 3), dtype=torch.float32, device=self.device)
+        faces = torch.empty((sum(nf), 3), dtype=torch.int64, device=self.device)
+        _lib.check(_lib.lib().thull_visual_hull_fetch_all(self.ctx, _ptr(verts), _ptr(faces), self.stream))
+        out, v0, f0 = [], 0, 0
+        for a, b in zip(nv, nf):
+            out.append((verts[v0:v0 + a], faces[f0:f0 + b]))
+            v0 += a
+            f0 += b
+        return out
 
     def mesh(self, i: int) -> tuple[torch.Tensor, torch.Tensor]:
         c = self.counts[i]
@@ -196,7 +214,8 @@ def visual_hull_batch(scenes: Sequence[dict], unique_verts: bool = True) -> list
     Each scene is a dict with the keyword arguments of visual_hull (masks,
     transforms, level, cube_corner_bfl, cube_length, masks_partial,
     transforms_convention).  Every level of every scene is carved in one
-    launch per octree level; results equal those of separate visual_hull calls."""
+    launch per octree level; results equal those of separate visual_hull calls.
+    The meshes are views into one vertex tensor and one face tensor."""
     _check(len(scenes) > 0, "len(scenes) > 0")
     keep: list = []
     structs = []
@@ -206,7 +225,7 @@ def visual_hull_batch(scenes: Sequence[dict], unique_verts: bool = True) -> list
         structs.append(_scene_struct(sc["masks"], sc["transforms"], sc["level"], sc["cube_corner_bfl"],
                                      sc["cube_length"], sc["masks_partial"], sc["transforms_convention"], keep))
     run = _Run(structs, device, unique_verts)
-    return [run.mesh(i) for i in range(len(structs))]
+    return run.meshes()
 
 
 def candidate_voxels_to_wireframes(candidate_voxels: Sequence[torch.Tensor], cube_corner_bfl: Sequence[float],
@@ -346,7 +365,7 @@ def gaussian_blur_batch(images: Sequence[torch.Tensor], kernel_size: int, sigma:
     for im in images:
         _check(im.device == device and im.dtype == in_dtype, "one device and dtype per batch")
         _check(im.dim() == 4 and im.size(3) == 1, "images [B,H,W,1]")
-        srcs.append(im[..., 0].contiguous())
+        srcs.append(im if im.is_contiguous() else im[..., 0].contiguous())  # [B,H,W,1] contiguous = [B,H,W]
         outs.append(torch.empty(tuple(im.shape), dtype=out_dtype, device=device))
         sizes.extend(int(v) for v in im.shape[:3])
     n = len(images)

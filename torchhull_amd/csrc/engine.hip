@@ -273,8 +273,8 @@ scene_starts_kernel(const uint64_t* word_begin,
     uint64_t vs = vtotal, ts = ttotal;
     if (first < nverts)
     {
-        vs = vbase != nullptr ? vbase[first] : 0;
         ts = tbase[first];
+        vs = vbase != nullptr ? vbase[first] : ts; // soup vertices sit at the triangle-vertex slots
     }
     out[3 * s + 0] = first;
     out[3 * s + 1] = vs;
@@ -439,10 +439,18 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i + 1), st));
         exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st);
+        // the children go out before the host learns their number (8 per parent at most), so the
+        // scatter overlaps the host's wait
+        uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(uint64_t(count) * 8, 1));
+        bool target = false; // a scene ends at level i + 1
+        for (int32_t s = 0; s < n; ++s)
+            target |= sd[s].level == i + 1;
+        launch_carve_scatter(list, count, mask, boff, children, i + 1, scenes, target ? cells : nullptr, st);
         THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         THULL_HIP(hipMemcpyAsync(ctx.stage->p + 1, dev_tests, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         THULL_HIP(hipStreamSynchronize(st));
         const uint64_t next = ctx.stage->p[0];
+        check_u32(next, "octree level size");
         ctx.level_tests.push_back(int64_t(ctx.stage->p[1]));
         ctx.level_children.push_back(int64_t(count) * 8);
         ctx.carve_parents.push_back(int64_t(count));
@@ -454,12 +462,6 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
             THULL_HIP(hipEventElapsedTime(&ms, ctx.event(2 * i), ctx.event(2 * i + 1)));
             ctx.carve_ms.push_back(ms);
         }
-        check_u32(next, "octree level size");
-        uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(next, 1));
-        bool target = false; // a scene ends at level i + 1
-        for (int32_t s = 0; s < n; ++s)
-            target |= sd[s].level == i + 1;
-        launch_carve_scatter(list, count, mask, boff, children, i + 1, scenes, target ? cells : nullptr, st);
         list = children;
         count = uint32_t(next);
         ctx.level_lists.push_back(list);
@@ -727,6 +729,29 @@ thull_visual_hull_fetch_mesh(thull_ctx* ctx, int32_t scene, float* verts, int64_
         if (k.num_faces > 0)
             THULL_HIP(hipMemcpyAsync(faces, c.faces + uint64_t(c.scene_tstart[scene]),
                                      3 * sizeof(int64_t) * uint64_t(k.num_faces), hipMemcpyDeviceToDevice, st));
+    });
+}
+
+int
+thull_visual_hull_fetch_all(thull_ctx* ctx, float* verts, int64_t* faces, void* stream)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr, "ctx is null");
+        Context& c = ctx->c;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        uint64_t nv = 0, nf = 0;
+        for (int32_t s = 0; s < c.n_scenes; ++s)
+        {
+            nv += uint64_t(c.counts[s].num_verts);
+            nf += uint64_t(c.counts[s].num_faces);
+        }
+        // scenes are stored one after the other in scene order, as the caller's buffers expect
+        if (nv > 0)
+            THULL_HIP(hipMemcpyAsync(verts, c.verts + 3 * uint64_t(c.scene_vstart[0]), 3 * sizeof(float) * nv,
+                                     hipMemcpyDeviceToDevice, st));
+        if (nf > 0)
+            THULL_HIP(hipMemcpyAsync(faces, c.faces + uint64_t(c.scene_tstart[0]), 3 * sizeof(int64_t) * nf,
+                                     hipMemcpyDeviceToDevice, st));
     });
 }
 

@@ -708,7 +708,7 @@ classify_entry(const SceneDev& s, uint64_t key, int32_t level_i, uint32_t& tests
     return classify_parent<PARTIAL>(s, key_x(key), key_y(key), key_z(key), level_i, level_i == s.level - 1, tests);
 }
 
-__global__ __launch_bounds__(kCarveThreads) void
+__global__ __launch_bounds__(kCarveThreads) __attribute__((amdgpu_waves_per_eu(4))) void
 carve_classify_kernel(const uint64_t* __restrict__ parents,
                       uint32_t n,
                       int32_t level_i,
