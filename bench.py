@@ -19,6 +19,7 @@ import json
 import math
 import os
 import platform
+import statistics
 import sys
 import threading
 import time
@@ -75,21 +76,27 @@ def timed(fn, steps: int, warmup: int):
     return time.perf_counter() - t0, out
 
 
-def carve_profile(run_fn) -> dict:
-    """Runs run_fn once with event timing of the carving classification launches; per-level records."""
+def carve_profile(run_fn, reps: int = 5) -> dict:
+    """Runs run_fn `reps` times with event timing of the carving classification launches; per-level
+    records with the median time over the runs."""
     ctx = _lib.context(torch.cuda.current_device())
     L = _lib.lib()
-    _lib.check(L.thull_profile_enable(ctx, 1))
-    try:
-        run_fn()
-        torch.cuda.synchronize()
-    finally:
-        _lib.check(L.thull_profile_enable(ctx, 0))
     n = 16
-    ms = (ctypes.c_float * n)()
-    par, tests, nxt = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
-    split = (ctypes.c_int32 * n)()
-    _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt, split))
+    runs = []
+    for _ in range(max(1, reps)):
+        _lib.check(L.thull_profile_enable(ctx, 1))
+        try:
+            run_fn()
+            torch.cuda.synchronize()
+        finally:
+            _lib.check(L.thull_profile_enable(ctx, 0))
+        ms = (ctypes.c_float * n)()
+        par, tests, nxt = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+        split = (ctypes.c_int32 * n)()
+        _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt, split))
+        runs.append((list(ms), list(par), list(tests), list(nxt), list(split)))
+    ms_med = [statistics.median(r[0][i] for r in runs) for i in range(n)]
+    _, par, tests, nxt, split = runs[-1]
     levels = []
     for i in range(n):
         if par[i] == 0:
@@ -98,9 +105,9 @@ def carve_profile(run_fn) -> dict:
         # test (early exit), children written (4 B)
         b = 4 * par[i] + 16 * tests[i] + 4 * nxt[i]
         kernel = "carve_classify_kernel" if split[i] == 1 else f"carve_classify_split_kernel<{split[i]}>"
-        levels.append(dict(step=i, level=i + 1, kernel=kernel, ms=float(ms[i]), parents=int(par[i]),
+        levels.append(dict(step=i, level=i + 1, kernel=kernel, ms=float(ms_med[i]), parents=int(par[i]),
                            camera_tests=int(tests[i]), children=int(nxt[i]), bytes=int(b)))
-    return dict(levels=levels)
+    return dict(levels=levels, reps=len(runs))
 
 
 def traffic_from_profile() -> dict | None:

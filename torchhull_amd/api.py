@@ -75,42 +75,40 @@ def _masks_for_abi(masks: torch.Tensor) -> tuple[torch.Tensor, int]:
     return m, code
 
 
-def _check_vh_args(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_length: float) -> None:
+def _check_vh_args(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_length: float) -> torch.Size:
+    """The argument checks of visual_hull.cpp:34-58; returns the masks' shape."""
+    ms, ts = masks.shape, transforms.shape
     _check(masks.device == transforms.device, "masks.device() == transforms.device()")
-    _check(masks.dim() == 4, "masks.dim() == 4")
-    _check(transforms.dim() == 3, "transforms.dim() == 3")
-    _check(transforms.size(1) == 4, "transforms.size(1) == 4")
-    _check(transforms.size(2) == 4, "transforms.size(2) == 4")
-    _check(masks.size(0) == transforms.size(0), "masks.size(0) == transforms.size(0)")
-    _check(masks.size(3) == 1, "masks.size(3) == 1")
+    _check(len(ms) == 4, "masks.dim() == 4")
+    _check(len(ts) == 3, "transforms.dim() == 3")
+    _check(ts[1] == 4, "transforms.size(1) == 4")
+    _check(ts[2] == 4, "transforms.size(2) == 4")
+    _check(ms[0] == ts[0], "masks.size(0) == transforms.size(0)")
+    _check(ms[3] == 1, "masks.size(3) == 1")
     _check(int(level) >= 0, "level >= 0")
     _check(float(cube_length) > 0.0, "cube_length > 0.f")
+    return ms
+
+
+_F3 = ctypes.c_float * 3
 
 
 def _scene_struct(masks: torch.Tensor, transforms: torch.Tensor, level: int, cube_corner_bfl: Sequence[float],
                   cube_length: float, masks_partial: bool, transforms_convention: str,
                   keep: list) -> _lib.Scene:
     _require_gpu(masks)
-    _check_vh_args(masks, transforms, level, cube_length)
+    ms = _check_vh_args(masks, transforms, level, cube_length)
     opengl = _convention(transforms_convention)
     m, code = _masks_for_abi(masks)
     T = transforms
     if T.dtype != torch.float32 or not T.is_contiguous():
         T = T.to(torch.float32).contiguous()  # fmaf arguments are float (visual_hull_cuda.cu:150-153)
-    keep.extend([m, T])
-    corner = [float(c) for c in cube_corner_bfl]
-    _check(len(corner) == 3, "len(cube_corner_bfl) == 3")
-    s = _lib.Scene()
-    s.masks = m.data_ptr()
-    s.mask_dtype = code
-    s.transforms = T.data_ptr()
-    s.B, s.H, s.W = int(masks.size(0)), int(masks.size(1)), int(masks.size(2))
-    s.level = int(level)
-    s.masks_partial = int(bool(masks_partial))
-    s.opengl = opengl
-    s.cube_corner_bfl[0], s.cube_corner_bfl[1], s.cube_corner_bfl[2] = corner
-    s.cube_length = float(cube_length)
-    return s
+    keep.append(m)
+    keep.append(T)
+    c = cube_corner_bfl
+    _check(len(c) == 3, "len(cube_corner_bfl) == 3")
+    return _lib.Scene(m.data_ptr(), code, T.data_ptr(), ms[0], ms[1], ms[2], int(level), int(bool(masks_partial)),
+                      opengl, _F3(float(c[0]), float(c[1]), float(c[2])), float(cube_length))
 
 
 class _Run:

@@ -446,7 +446,7 @@ blur_job_bytes(size_t n)
 
 template <typename Ti, typename To, int RB>
 static void
-run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm,
+run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, bool sparse, const To* weights, To norm,
             uint8_t* flags, hipStream_t st)
 {
     const int32_t ts = k / 2;
@@ -474,7 +474,7 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
     {
         THULL_CHECK(eb < (uint64_t(1) << 31), "blur batch too large");
         THULL_HIP(hipMemsetAsync(flags, 0, fbytes, st));
-        THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
+        up.upload(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), st);
         if (vec)
         {
             if constexpr (std::is_same_v<Ti, float>)
@@ -483,8 +483,6 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
         else
             blur_edge_flags_kernel<Ti, RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
         THULL_LAUNCH_CHECK();
-        // the tile pass uses its own block layout; the host table must stay valid until the copy ran
-        THULL_HIP(hipStreamSynchronize(st));
     }
     for (auto& j : jobs)
     {
@@ -495,7 +493,8 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
     if (pb == 0)
         return;
     THULL_CHECK(pb < (uint64_t(1) << 31), "blur batch too large");
-    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), hipMemcpyHostToDevice, st));
+    // the tile pass has its own block layout: a second table, copied after the edge pass read the first
+    up.upload(djobs, jobs.data(), jobs.size() * sizeof(BlurJob), st);
     const int32_t n = int32_t(jobs.size());
     const int32_t h = k / 2;
     const size_t lds = size_t(kBlurBX + 2 * h) * (RB + 2 * h) * sizeof(To);
@@ -517,40 +516,38 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, cons
     else
         blur_tile_kernel<Ti, To, RB, 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights, norm);
     THULL_LAUNCH_CHECK();
-    // the job table is pageable host memory read by an async copy
-    THULL_HIP(hipStreamSynchronize(st));
 }
 
 template <typename Ti, typename To>
 static void
-run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const To* weights, To norm, uint8_t* flags,
-         hipStream_t st)
+run_blur(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, bool sparse, const To* weights, To norm,
+         uint8_t* flags, hipStream_t st)
 {
     const int32_t h = k / 2;
     if (size_t(kBlurBX + 2 * h) * (64 + 2 * h) * sizeof(To) <= 48 * 1024)
-        run_blur_rb<Ti, To, 64>(jobs, djobs, k, sparse, weights, norm, flags, st);
+        run_blur_rb<Ti, To, 64>(jobs, djobs, up, k, sparse, weights, norm, flags, st);
     else if (size_t(kBlurBX + 2 * h) * (16 + 2 * h) * sizeof(To) <= 48 * 1024)
-        run_blur_rb<Ti, To, 16>(jobs, djobs, k, sparse, weights, norm, flags, st);
+        run_blur_rb<Ti, To, 16>(jobs, djobs, up, k, sparse, weights, norm, flags, st);
     else
         throw Error("kernel_size too large for the LDS tile");
 }
 
 template <typename Ti>
 static void
-run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, const void* weights, double norm,
-             int32_t out_dtype, uint8_t* flags, hipStream_t st)
+run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, bool sparse, const void* weights,
+             double norm, int32_t out_dtype, uint8_t* flags, hipStream_t st)
 {
     switch (out_dtype)
     {
     case kF16:
-        run_blur<Ti, __half>(jobs, djobs, k, sparse, static_cast<const __half*>(weights), __float2half(float(norm)),
+        run_blur<Ti, __half>(jobs, djobs, up, k, sparse, static_cast<const __half*>(weights), __float2half(float(norm)),
                              flags, st);
         break;
     case kF32:
-        run_blur<Ti, float>(jobs, djobs, k, sparse, static_cast<const float*>(weights), float(norm), flags, st);
+        run_blur<Ti, float>(jobs, djobs, up, k, sparse, static_cast<const float*>(weights), float(norm), flags, st);
         break;
     case kF64:
-        run_blur<Ti, double>(jobs, djobs, k, sparse, static_cast<const double*>(weights), norm, flags, st);
+        run_blur<Ti, double>(jobs, djobs, up, k, sparse, static_cast<const double*>(weights), norm, flags, st);
         break;
     default:
         throw Error("gaussian_blur: output dtype must be floating point");
@@ -563,7 +560,7 @@ run_blur_out(std::vector<BlurJob>& jobs, BlurJob* djobs, int k, bool sparse, con
 void
 launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int k, bool sparse,
                            const void* weights, double norm, int32_t out_dtype, uint8_t* flags, void* job_buf,
-                           hipStream_t st)
+                           Uploader& up, hipStream_t st)
 {
     std::vector<BlurJob> jobs;
     for (int32_t i = 0; i < n; ++i)
@@ -588,28 +585,28 @@ launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, i
     {
     case kBool:
     case kU8:
-        run_blur_out<uint8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<uint8_t>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI8:
-        run_blur_out<int8_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<int8_t>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI16:
-        run_blur_out<int16_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<int16_t>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI32:
-        run_blur_out<int32_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<int32_t>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kI64:
-        run_blur_out<int64_t>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<int64_t>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF16:
-        run_blur_out<__half>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<__half>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF32:
-        run_blur_out<float>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<float>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     case kF64:
-        run_blur_out<double>(jobs, djobs, k, sparse, weights, norm, out_dtype, flags, st);
+        run_blur_out<double>(jobs, djobs, up, k, sparse, weights, norm, out_dtype, flags, st);
         break;
     default:
         throw Error("gaussian_blur: unsupported input dtype");

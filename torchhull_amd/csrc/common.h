@@ -154,21 +154,46 @@ lerp_ref(float v0, float v1, float t)
 // operations without them give the same bits; the reciprocal does not depend
 // on the numerator, so both quotients share it.  Other operands take the
 // plain division.
+//
+// A NaN operand gives a NaN quotient on both routes, so the range test may
+// ignore NaN operands (as fmaxf / fminf do): only the NaN-ness of a quotient
+// is used downstream.
+__device__ __forceinline__ void
+div2_unchecked(float nx, float ny, float d, float& qx, float& qy)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(fmaf(-d, r, 1.0f), r, r);
+    float a = nx * r, b = ny * r;
+    a = fmaf(fmaf(-d, a, nx), r, a);
+    b = fmaf(fmaf(-d, b, ny), r, b);
+    qx = fmaf(fmaf(-d, a, nx), r, a);
+    qy = fmaf(fmaf(-d, b, ny), r, b);
+}
+
+// Largest and smallest magnitude of three operands (NaN ignored).
+__device__ __forceinline__ float
+max_abs3(float a, float b, float c)
+{
+    return fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(c));
+}
+__device__ __forceinline__ float
+min_abs3(float a, float b, float c)
+{
+    return fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));
+}
+
+// True when every operand magnitude between lo and hi lies in [2^-40, 2^40].
+__device__ __forceinline__ bool
+div2_in_range(float hi, float lo)
+{
+    return hi <= 0x1p40f && lo >= 0x1p-40f;
+}
+
 __device__ __forceinline__ void
 div2_shared(float nx, float ny, float d, float& qx, float& qy)
 {
-    const float hi = fmaxf(fmaxf(fabsf(nx), fabsf(ny)), fabsf(d));
-    const float lo = fminf(fminf(fabsf(nx), fabsf(ny)), fabsf(d));
-    if (hi <= 0x1p40f && lo >= 0x1p-40f)
-    {
-        float r = __builtin_amdgcn_rcpf(d);
-        r = fmaf(fmaf(-d, r, 1.0f), r, r);
-        float a = nx * r, b = ny * r;
-        a = fmaf(fmaf(-d, a, nx), r, a);
-        b = fmaf(fmaf(-d, b, ny), r, b);
-        qx = fmaf(fmaf(-d, a, nx), r, a);
-        qy = fmaf(fmaf(-d, b, ny), r, b);
-    }
+    if (div2_in_range(max_abs3(nx, ny, d), min_abs3(nx, ny, d)))
+        div2_unchecked(nx, ny, d, qx, qy);
     else
     {
         qx = nx / d;
@@ -296,6 +321,24 @@ find_segment_rising(F begin, int32_t n, uint64_t v)
     return s;
 }
 
+// The scene descriptor with every 32-bit word read from the first active
+// lane: for a descriptor that all active lanes share, the compiler then keeps
+// it, and what it loads through its pointers, in scalar registers.
+__device__ __forceinline__ SceneDev
+wave_uniform(const SceneDev& s)
+{
+    constexpr int kWords = sizeof(SceneDev) / 4;
+    static_assert(sizeof(SceneDev) % 4 == 0, "SceneDev is a whole number of words");
+    uint32_t w[kWords];
+    __builtin_memcpy(w, &s, sizeof(SceneDev));
+#pragma unroll
+    for (int i = 0; i < kWords; ++i)
+        w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    SceneDev u;
+    __builtin_memcpy(&u, w, sizeof(SceneDev));
+    return u;
+}
+
 // Quotient of a 64-bit value by a 32-bit divisor, through the 32-bit divide
 // when the dividend fits (the common case; 64-bit division is a long
 // software sequence).
@@ -310,6 +353,18 @@ div_u64_u32(uint64_t a, uint32_t b)
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
+
+// Host-to-device copies of small host tables (job lists, scene descriptors)
+// in stream order.  The implementation stages the bytes in pinned memory when
+// upload() is called, so the copy neither blocks the host nor reads the host
+// table later (a copy from pageable memory waits for the stream's earlier work).
+struct Uploader
+{
+    virtual void upload(void* dst, const void* src, size_t bytes, hipStream_t st) = 0;
+
+protected:
+    ~Uploader() = default;
+};
 
 inline unsigned
 blocks_for(int64_t n, int threads)

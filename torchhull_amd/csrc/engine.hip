@@ -8,6 +8,7 @@
 // level, the vertex count, the mesh sizes).
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -128,6 +129,166 @@ struct HostStage
     ~HostStage() { (void)hipHostFree(p); }
 };
 
+// Pinned staging chunks for the host-to-device table copies.  upload() copies
+// the bytes into the current chunk and queues the device copy; seal() marks
+// the end of a call with an event, and a chunk is reused only after its event
+// completed, so a queued copy never sees its bytes overwritten.  A call that
+// needs more room than its chunk has left seals it and takes another.
+class UploadRing final : public Uploader
+{
+public:
+    UploadRing() = default;
+    UploadRing(const UploadRing&) = delete;
+    UploadRing& operator=(const UploadRing&) = delete;
+    ~UploadRing()
+    {
+        for (auto& c : chunks_)
+        {
+            (void)hipEventDestroy(c.ev);
+            (void)hipHostFree(c.p);
+        }
+    }
+
+    void
+    upload(void* dst, const void* src, size_t bytes, hipStream_t st) override
+    {
+        if (bytes == 0)
+            return;
+        const size_t need = (bytes + 255) & ~size_t(255);
+        if (cur_ < 0 || chunks_[cur_].used + need > chunks_[cur_].cap)
+        {
+            seal(st);
+            cur_ = take(need);
+        }
+        Chunk& c = chunks_[cur_];
+        char* p = c.p + c.used;
+        c.used += need;
+        std::memcpy(p, src, bytes);
+        THULL_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st));
+    }
+
+    // The copies queued so far on `st` are the last ones reading the current chunk.
+    void
+    seal(hipStream_t st)
+    {
+        if (cur_ < 0)
+            return;
+        Chunk& c = chunks_[cur_];
+        if (c.used > 0)
+        {
+            THULL_HIP(hipEventRecord(c.ev, st));
+            c.recorded = true;
+        }
+        cur_ = -1;
+    }
+
+private:
+    struct Chunk
+    {
+        char* p = nullptr;
+        size_t cap = 0, used = 0;
+        hipEvent_t ev = nullptr;
+        bool recorded = false;
+    };
+    static constexpr size_t kChunk = size_t(1) << 20;
+    std::vector<Chunk> chunks_;
+    int cur_ = -1;
+
+    int
+    take(size_t need)
+    {
+        for (size_t i = 0; i < chunks_.size(); ++i)
+        {
+            Chunk& c = chunks_[i];
+            if (c.cap < need)
+                continue;
+            if (c.recorded)
+            {
+                const hipError_t q = hipEventQuery(c.ev);
+                if (q == hipErrorNotReady)
+                    continue;
+                THULL_HIP(q);
+                c.recorded = false;
+            }
+            c.used = 0;
+            return int(i);
+        }
+        Chunk c;
+        c.cap = std::max(kChunk, need);
+        THULL_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.p), c.cap));
+        const hipError_t e = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
+        if (e != hipSuccess)
+        {
+            (void)hipHostFree(c.p);
+            THULL_HIP(e);
+        }
+        chunks_.push_back(c);
+        return int(chunks_.size()) - 1;
+    }
+};
+
+// Coherent pinned words that scan kernels write their totals to (ScanNotify):
+// the host learns a level's size as soon as its scan ends, while the kernels
+// queued after the scan still run, instead of waiting for the whole stream.
+class HostNotify
+{
+public:
+    HostNotify()
+    {
+        THULL_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), 4 * sizeof(unsigned long long),
+                                hipHostMallocCoherent | hipHostMallocMapped));
+        const hipError_t e = hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_), host_, 0);
+        if (e != hipSuccess)
+        {
+            (void)hipHostFree(host_);
+            THULL_HIP(e);
+        }
+        host_[0] = host_[1] = host_[2] = 0;
+    }
+    HostNotify(const HostNotify&) = delete;
+    HostNotify& operator=(const HostNotify&) = delete;
+    ~HostNotify() { (void)hipHostFree(host_); }
+
+    // The notification for the next scan (a fresh sequence number).
+    ScanNotify
+    next(const unsigned long long* extra)
+    {
+        return ScanNotify{ dev_, extra, ++seq_ };
+    }
+
+    // Waits for the notification `n` (queued on `st`); false when the stream went idle without it,
+    // and the caller then reads the values from device memory.
+    bool
+    wait(const ScanNotify& n, hipStream_t st, unsigned long long (&out)[2]) const
+    {
+        volatile unsigned long long* v = host_;
+        for (uint32_t i = 1;; ++i)
+        {
+            if (v[2] == n.seq)
+            {
+                std::atomic_thread_fence(std::memory_order_acquire);
+                out[0] = v[0];
+                out[1] = v[1];
+                return true;
+            }
+            if ((i & 255u) == 0u)
+            {
+                const hipError_t q = hipStreamQuery(st);
+                if (q == hipSuccess)
+                    return v[2] == n.seq ? wait(n, st, out) : false;
+                if (q != hipErrorNotReady)
+                    THULL_HIP(q);
+            }
+            __builtin_ia32_pause();
+        }
+    }
+
+private:
+    unsigned long long* host_ = nullptr;
+    unsigned long long* dev_ = nullptr;
+    unsigned long long seq_ = 0;
+};
+
 } // namespace
 
 struct Context
@@ -135,6 +296,8 @@ struct Context
     int device = 0;
     Pool pool;
     HostStage* stage = nullptr;
+    UploadRing upload;
+    HostNotify notify;
     hipStream_t last_stream = nullptr;
     bool have_last = false;
 
@@ -196,8 +359,13 @@ struct Context
     begin(hipStream_t st)
     {
         THULL_HIP(hipSetDevice(device));
-        if (have_last && last_stream != st)
-            THULL_HIP(hipStreamSynchronize(last_stream));
+        if (have_last)
+        {
+            // the staging chunk of the previous call is free once its stream passes this point
+            upload.seal(last_stream);
+            if (last_stream != st)
+                THULL_HIP(hipStreamSynchronize(last_stream));
+        }
         last_stream = st;
         have_last = true;
         pool.reset();
@@ -260,13 +428,14 @@ scene_starts_kernel(const uint64_t* word_begin,
                     uint64_t nverts,
                     const uint32_t* vbase,
                     const uint32_t* tbase,
-                    uint64_t vtotal,
-                    uint64_t ttotal,
+                    const unsigned long long* totals, // triangle vertices, output vertices (unique meshes)
+                    bool unique,
                     unsigned long long* out)
 {
     const int32_t s = blockIdx.x * 256 + threadIdx.x;
     if (s > n)
         return;
+    const uint64_t ttotal = totals[0], vtotal = unique ? totals[1] : ttotal;
     uint64_t first = nverts;
     if (s < n && word_begin[s] < nwords)
         first = bitmap_prefix[word_begin[s]];
@@ -395,13 +564,12 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     for (const auto& r : ireq)
         carry_floats += integral_carry_floats(r.B, r.H, r.W);
     launch_integral_images_padded(ireq.data(), int32_t(ireq.size()), pool.alloc<float>(std::max<uint64_t>(carry_floats, 1)),
-                                  pool.alloc_bytes(integral_job_bytes(std::max<size_t>(ireq.size(), 1))), st);
+                                  pool.alloc_bytes(integral_job_bytes(std::max<size_t>(ireq.size(), 1))), ctx.upload, st);
     word_begin[n] = total_words;
     ctx.scenes_host = sd;
     SceneDev* scenes = pool.alloc<SceneDev>(n);
-    THULL_HIP(hipMemcpyAsync(scenes, sd.data(), n * sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    ctx.upload.upload(scenes, sd.data(), n * sizeof(SceneDev), st);
     unsigned long long* dev_total = pool.alloc<unsigned long long>(2);
-    unsigned long long* dev_tests = pool.alloc<unsigned long long>(1);
 
     // ---- level 0: one root cell per scene that carves at all ----
     std::vector<int32_t> live;
@@ -413,7 +581,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
     for (size_t k = 0; k < live.size(); ++k)
         roots[k] = pack_key(uint32_t(live[k]), 0, 0, 0);
     if (!roots.empty())
-        THULL_HIP(hipMemcpyAsync(list, roots.data(), roots.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        ctx.upload.upload(list, roots.data(), roots.size() * sizeof(uint64_t), st);
     uint32_t count = uint32_t(live.size());
     ctx.level_lists.push_back(list);
     ctx.level_sizes.push_back(count);
@@ -424,6 +592,10 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         THULL_HIP(hipMemsetAsync(cells, 0, total_cell_words * sizeof(uint32_t), st));
     launch_mark_root_cells(scenes, n, cells, st);
 
+    // camera-test counters of every level, cleared once
+    unsigned long long* dev_tests = pool.alloc<unsigned long long>(std::max(max_level, 1));
+    THULL_HIP(hipMemsetAsync(dev_tests, 0, std::max(max_level, 1) * sizeof(unsigned long long), st));
+
     // ---- level loop (VH:634-751) ----
     for (int32_t i = 0; i < max_level && count > 0; ++i)
     {
@@ -432,26 +604,34 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         uint32_t* bcount = pool.alloc<uint32_t>(nb);
         uint32_t* boff = pool.alloc<uint32_t>(nb);
         uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nb) + 1);
-        THULL_HIP(hipMemsetAsync(dev_tests, 0, sizeof(unsigned long long), st));
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i), st));
-        const int32_t split = launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests, st);
+        const int32_t split = launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests + i, st);
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i + 1), st));
-        exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st);
-        // the children go out before the host learns their number (8 per parent at most), so the
-        // scatter overlaps the host's wait
+        // the scan sends the next level's size (and the camera tests) to the host; the children go out
+        // before the host learns their number (8 per parent at most), so the scatter runs while the host
+        // queues the next level
+        const ScanNotify note = ctx.notify.next(dev_tests + i);
+        exclusive_scan(LoadU32{ bcount }, nb, boff, sums, dev_total, st, note);
         uint64_t* children = pool.alloc<uint64_t>(std::max<uint64_t>(uint64_t(count) * 8, 1));
         bool target = false; // a scene ends at level i + 1
         for (int32_t s = 0; s < n; ++s)
             target |= sd[s].level == i + 1;
         launch_carve_scatter(list, count, mask, boff, children, i + 1, scenes, target ? cells : nullptr, st);
-        THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        THULL_HIP(hipMemcpyAsync(ctx.stage->p + 1, dev_tests, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        THULL_HIP(hipStreamSynchronize(st));
-        const uint64_t next = ctx.stage->p[0];
+        unsigned long long got[2];
+        if (!ctx.notify.wait(note, st, got))
+        {
+            THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            THULL_HIP(
+                hipMemcpyAsync(ctx.stage->p + 1, dev_tests + i, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            THULL_HIP(hipStreamSynchronize(st));
+            got[0] = ctx.stage->p[0];
+            got[1] = ctx.stage->p[1];
+        }
+        const uint64_t next = got[0];
         check_u32(next, "octree level size");
-        ctx.level_tests.push_back(int64_t(ctx.stage->p[1]));
+        ctx.level_tests.push_back(int64_t(got[1]));
         ctx.level_children.push_back(int64_t(count) * 8);
         ctx.carve_parents.push_back(int64_t(count));
         ctx.carve_next.push_back(int64_t(next));
@@ -459,6 +639,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         if (ctx.profile)
         {
             float ms = 0.0f;
+            THULL_HIP(hipEventSynchronize(ctx.event(2 * i + 1)));
             THULL_HIP(hipEventElapsedTime(&ms, ctx.event(2 * i), ctx.event(2 * i + 1)));
             ctx.carve_ms.push_back(ms);
         }
@@ -471,21 +652,22 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
 
     // ---- corner vertices (VH:770-825): leaf cells (marked above), then their corners by dilation ----
     uint64_t* dev_word_begin = pool.alloc<uint64_t>(n + 1);
-    THULL_HIP(hipMemcpyAsync(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    ctx.upload.upload(dev_word_begin, word_begin.data(), (n + 1) * sizeof(uint64_t), st);
     uint32_t* bitmap = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
     uint32_t* prefix = pool.alloc<uint32_t>(std::max<uint64_t>(total_words, 1));
     // dilation + tile popcount sums, scan of the sums, then word prefixes + vertex keys
     const uint32_t wblocks = scan_blocks(total_words);
     uint32_t* wsums = pool.alloc<uint32_t>(wblocks + 1);
     launch_dilate_cells(cells, bitmap, total_words, dev_word_begin, scenes, n, wsums, st);
+    uint64_t nverts = 0;
     if (wblocks > 0)
     {
-        scan_block_sums_kernel<><<<1, 1024, 0, st>>>(wsums, wblocks, dev_total);
+        const ScanNotify note = ctx.notify.next(nullptr);
+        scan_block_sums_kernel<><<<1, 1024, 0, st>>>(wsums, wblocks, dev_total, note);
         THULL_LAUNCH_CHECK();
+        unsigned long long got[2];
+        nverts = ctx.notify.wait(note, st, got) ? got[0] : ctx.read_u64(dev_total, st);
     }
-    else
-        THULL_HIP(hipMemsetAsync(dev_total, 0, sizeof(unsigned long long), st));
-    const uint64_t nverts = ctx.read_u64(dev_total, st);
     check_u32(nverts, "sparse field size");
     uint64_t* vertices = pool.alloc<uint64_t>(std::max<uint64_t>(nverts, 1));
     float* values = pool.alloc<float>(std::max<uint64_t>(nverts, 1));
@@ -515,7 +697,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         grids[s] = g;
     }
     McGrid* dgrids = pool.alloc<McGrid>(n);
-    THULL_HIP(hipMemcpyAsync(dgrids, grids.data(), n * sizeof(McGrid), hipMemcpyHostToDevice, st));
+    ctx.upload.upload(dgrids, grids.data(), n * sizeof(McGrid), st);
     McPlan plan{};
     plan.grids = dgrids;
     plan.vertices = vertices;
@@ -548,21 +730,28 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
             launch_mc_edges(plan, info, emask, st);
             exclusive_scan(LoadPopcountMask{ emask }, nverts, vbase, sums, dev_total + 1, st);
         }
-        else
-            THULL_HIP(hipMemsetAsync(dev_total + 1, 0, sizeof(unsigned long long), st));
-        THULL_HIP(hipMemcpyAsync(ctx.stage->p, dev_total, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        THULL_HIP(hipStreamSynchronize(st));
-        ttotal = ctx.stage->p[0];
-        vtotal = unique ? ctx.stage->p[1] : ttotal;
-        check_u32(ttotal, "triangle vertex count");
         unsigned long long* dstarts = pool.alloc<unsigned long long>(3 * (n + 1));
         scene_starts_kernel<<<blocks_for(n + 1, 256), 256, 0, st>>>(dev_word_begin, n, total_words, prefix, nverts,
-                                                                    vbase, tbase, vtotal, ttotal, dstarts);
+                                                                    vbase, tbase, dev_total, unique, dstarts);
         THULL_LAUNCH_CHECK();
-        THULL_HIP(hipMemcpyAsync(starts.data(), dstarts, starts.size() * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, st));
-        std::vector<uint32_t> vstart(n + 1), tstart(n + 1);
+        // one wait for the two totals and the scene starts
+        const size_t words = 2 + starts.size();
+        std::vector<unsigned long long> big;
+        unsigned long long* h = ctx.stage->p;
+        if (words > size_t(HostStage::kSlots))
+        {
+            big.resize(words);
+            h = big.data();
+        }
+        THULL_HIP(hipMemcpyAsync(h, dev_total, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        THULL_HIP(hipMemcpyAsync(h + 2, dstarts, starts.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 st));
         THULL_HIP(hipStreamSynchronize(st));
+        ttotal = h[0];
+        vtotal = unique ? h[1] : ttotal;
+        check_u32(ttotal, "triangle vertex count");
+        std::copy(h + 2, h + words, starts.begin());
+        std::vector<uint32_t> vstart(n + 1), tstart(n + 1);
         for (int32_t s = 0; s <= n; ++s)
         {
             vstart[s] = uint32_t(starts[3 * s + 1]);
@@ -570,8 +759,8 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         }
         uint32_t* dv = pool.alloc<uint32_t>(n + 1);
         uint32_t* dt = pool.alloc<uint32_t>(n + 1);
-        THULL_HIP(hipMemcpyAsync(dv, vstart.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        THULL_HIP(hipMemcpyAsync(dt, tstart.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        ctx.upload.upload(dv, vstart.data(), (n + 1) * sizeof(uint32_t), st);
+        ctx.upload.upload(dt, tstart.data(), (n + 1) * sizeof(uint32_t), st);
         verts = pool.alloc<float>(3 * std::max<uint64_t>(vtotal, 1));
         faces = pool.alloc<int64_t>(std::max<uint64_t>(ttotal, 1));
         if (unique)
@@ -579,8 +768,6 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         launch_mc_faces(plan, info, tbase, emask, vbase, dv, dt, unique, verts, faces, st);
         ctx.scene_vstart = vstart;
         ctx.scene_tstart = tstart;
-        // pageable host vectors above were copied asynchronously: wait before they go out of scope
-        THULL_HIP(hipStreamSynchronize(st));
     }
     else
     {
@@ -864,7 +1051,7 @@ mc_run_common(Context& c, thull::McPlan plan, thull::McGrid grid, bool unique, h
     using namespace thull;
     Pool& pool = c.pool;
     thull::McGrid* dgrid = pool.alloc<McGrid>(1);
-    THULL_HIP(hipMemcpyAsync(dgrid, &grid, sizeof(McGrid), hipMemcpyHostToDevice, st));
+    c.upload.upload(dgrid, &grid, sizeof(McGrid), st);
     plan.grids = dgrid;
     c.mc_verts = nullptr;
     c.mc_faces = nullptr;
@@ -1007,7 +1194,7 @@ thull_marching_cubes_sparse_run(thull_ctx* ctx,
         {
             Pool& pool = c.pool;
             McGrid* dgrid = pool.alloc<McGrid>(1);
-            THULL_HIP(hipMemcpyAsync(dgrid, &grid, sizeof(McGrid), hipMemcpyHostToDevice, st));
+            c.upload.upload(dgrid, &grid, sizeof(McGrid), st);
             uint32_t* bitmap = pool.alloc<uint32_t>(nwords);
             uint32_t* prefix = pool.alloc<uint32_t>(nwords);
             THULL_HIP(hipMemsetAsync(bitmap, 0, nwords * sizeof(uint32_t), st));
@@ -1181,7 +1368,7 @@ thull_gaussian_blur(thull_ctx* ctx,
         if (is_sparse && k > 1)
             flags = c.pool.alloc<uint8_t>(uint64_t(blur_tile_flag_bytes(B, H, W, k)));
         launch_gaussian_blur_batch(&im, 1, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype, flags,
-                                   c.pool.alloc_bytes(blur_job_bytes(1)), st);
+                                   c.pool.alloc_bytes(blur_job_bytes(1)), c.upload, st);
     });
 }
 
@@ -1229,7 +1416,7 @@ thull_gaussian_blur_batch(thull_ctx* ctx,
         }
         uint8_t* flags = fbytes ? c.pool.alloc<uint8_t>(fbytes) : nullptr;
         launch_gaussian_blur_batch(ims.data(), n, in_dtype, k, is_sparse, it->second.dev, it->second.norm, out_dtype,
-                                   flags, c.pool.alloc_bytes(blur_job_bytes(std::max(n, 1))), st);
+                                   flags, c.pool.alloc_bytes(blur_job_bytes(std::max(n, 1))), c.upload, st);
     });
 }
 
@@ -1244,7 +1431,7 @@ thull_integral_image(thull_ctx* ctx, const void* masks, int32_t dtype, int64_t B
         ctx->c.begin(st);
         float* carry = ctx->c.pool.alloc<float>(std::max<uint64_t>(thull::integral_carry_floats(B, H, W), 1));
         thull::launch_integral_image(masks, dtype, B, H, W, out, carry,
-                                     ctx->c.pool.alloc_bytes(thull::integral_job_bytes(1)), st);
+                                     ctx->c.pool.alloc_bytes(thull::integral_job_bytes(1)), ctx->c.upload, st);
     });
 }
 

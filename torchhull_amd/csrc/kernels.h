@@ -14,7 +14,7 @@ namespace thull
 size_t integral_job_bytes(size_t njobs);
 uint64_t integral_carry_floats(int64_t B, int64_t H, int64_t W);
 void launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, float* carry,
-                           void* job_buf, hipStream_t st);
+                           void* job_buf, Uploader& up, hipStream_t st);
 // The same integral images, for many mask tensors in one launch per pass, in
 // the padded layout carving reads (integral_pitch in common.h): the zero row
 // and column make the zero padding of image_utils.h:84-97 branch-free.
@@ -26,7 +26,8 @@ struct IntegralRequest
     int64_t B, H, W;
     float* padded;
 };
-void launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, hipStream_t st);
+void launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, Uploader& up,
+                                   hipStream_t st);
 
 // ---- octree carving (vh_kernels.hip) ----
 constexpr int kCarveThreads = 256;
@@ -138,6 +139,6 @@ size_t blur_job_bytes(size_t n);
 // `flags`: sum over images of blur_tile_flag_bytes(B, H, W, k) bytes (sparse, k > 1).
 void launch_gaussian_blur_batch(const BlurImage* imgs, int32_t n, int32_t in_dtype, int kernel_size, bool sparse,
                                 const void* weights, double norm, int32_t out_dtype, uint8_t* flags, void* job_buf,
-                                hipStream_t st);
+                                Uploader& up, hipStream_t st);
 
 } // namespace thull

@@ -87,10 +87,21 @@ scan_reduce_kernel(L load, uint64_t n, uint32_t* block_sums)
         block_sums[blockIdx.x] = total;
 }
 
+// Host notification of a scan's total: `slot` (3 words of coherent pinned
+// host memory, device address) receives the total and a copy of *extra (0
+// without it), then `seq` with a system-scope release store, so a host that
+// sees seq in slot[2] reads the two words written before it.
+struct ScanNotify
+{
+    unsigned long long* slot = nullptr;
+    const unsigned long long* extra = nullptr;
+    unsigned long long seq = 0;
+};
+
 // Single-block exclusive scan of the block sums; writes the grand total (64-bit).
 template <typename Tag = void>
 __global__ __launch_bounds__(1024) void
-scan_block_sums_kernel(uint32_t* sums, uint32_t nblocks, unsigned long long* total_out)
+scan_block_sums_kernel(uint32_t* sums, uint32_t nblocks, unsigned long long* total_out, ScanNotify note = {})
 {
     __shared__ unsigned long long lds[1024 / 64];
     const uint32_t per = (nblocks + 1023) / 1024;
@@ -127,7 +138,15 @@ scan_block_sums_kernel(uint32_t* sums, uint32_t nblocks, unsigned long long* tot
             run += v;
         }
     if (threadIdx.x == 0)
+    {
         *total_out = total;
+        if (note.slot != nullptr)
+        {
+            note.slot[0] = total;
+            note.slot[1] = note.extra != nullptr ? *note.extra : 0ull;
+            __hip_atomic_store(note.slot + 2, note.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 template <typename L>
@@ -190,19 +209,22 @@ scan_blocks(uint64_t n)
     return uint32_t((n + kScanTile - 1) / kScanTile);
 }
 
+// `note` (optional, n > 0) also sends the total to the host.
 template <typename L>
 void
-exclusive_scan(L load, uint64_t n, uint32_t* out, uint32_t* block_sums, unsigned long long* total_dev, hipStream_t st)
+exclusive_scan(L load, uint64_t n, uint32_t* out, uint32_t* block_sums, unsigned long long* total_dev, hipStream_t st,
+               ScanNotify note = {})
 {
     const uint32_t nb = scan_blocks(n);
     if (nb == 0)
     {
+        THULL_CHECK(note.slot == nullptr, "scan notification of an empty scan");
         THULL_HIP(hipMemsetAsync(total_dev, 0, sizeof(unsigned long long), st));
         return;
     }
     scan_reduce_kernel<<<nb, kScanThreads, 0, st>>>(load, n, block_sums);
     THULL_LAUNCH_CHECK();
-    scan_block_sums_kernel<><<<1, 1024, 0, st>>>(block_sums, nb, total_dev);
+    scan_block_sums_kernel<><<<1, 1024, 0, st>>>(block_sums, nb, total_dev, note);
     THULL_LAUNCH_CHECK();
     scan_downsweep_kernel<<<nb, kScanThreads, 0, st>>>(load, n, block_sums, out);
     THULL_LAUNCH_CHECK();

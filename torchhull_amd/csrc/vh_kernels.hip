@@ -296,7 +296,8 @@ integral_job_bytes(size_t njobs)
 }
 
 static void
-integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* carry, bool border, hipStream_t st)
+integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* carry, bool border, Uploader& up,
+                   hipStream_t st)
 {
     uint64_t cols = 0, strips = 0, cams = 0, coff = 0;
     for (auto& j : jobs)
@@ -315,7 +316,7 @@ integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* ca
     if (strips == 0)
         return;
     THULL_CHECK(cols < (uint64_t(1) << 31) && strips < (uint64_t(1) << 31), "integral image batch too large");
-    THULL_HIP(hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(IntegralJob), hipMemcpyHostToDevice, st));
+    up.upload(djobs, jobs.data(), jobs.size() * sizeof(IntegralJob), st);
     const int32_t n = int32_t(jobs.size());
     if (border)
     {
@@ -330,23 +331,22 @@ integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* ca
     }
     integral_strip_kernel<<<unsigned(strips), 256, 0, st>>>(djobs, n);
     THULL_LAUNCH_CHECK();
-    // the job table is pageable host memory read by an async copy
-    THULL_HIP(hipStreamSynchronize(st));
 }
 
 void
 launch_integral_image(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, float* out, float* carry,
-                      void* job_buf, hipStream_t st)
+                      void* job_buf, Uploader& up, hipStream_t st)
 {
     if (B == 0 || H == 0 || W == 0)
         return;
     THULL_CHECK(dtype == kF32 || dtype == kU8 || dtype == kBool, "integral image: unsupported mask element type");
     std::vector<IntegralJob> jobs(1, make_job(masks, dtype, B, H, W, out, H * W, W));
-    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, false, st);
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, false, up, st);
 }
 
 void
-launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, hipStream_t st)
+launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carry, void* job_buf, Uploader& up,
+                              hipStream_t st)
 {
     std::vector<IntegralJob> jobs;
     for (int32_t i = 0; i < n; ++i)
@@ -359,7 +359,7 @@ launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carr
         const int64_t ld = integral_pitch(r.W);
         jobs.push_back(make_job(r.masks, r.dtype, r.B, r.H, r.W, r.padded + ld + 4, (r.H + 1) * ld, ld));
     }
-    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, true, st);
+    integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, true, up, st);
 }
 
 // ===========================================================================
@@ -428,14 +428,18 @@ box_merge(const Box& a, const Box& b)
                 fmaxf(a.maxy, b.maxy) };
 }
 
-// clamp(int64(roundf(v)), 0, hi) for the full-mask test: the clamp bounds are
+// clamp(int64(roundf(v)), 0, hi) of the window bounds: the clamp bounds are
 // integers and roundf is monotone, so clamping first gives the same integer,
 // NaN included (both map to 0).  For the clamped c in [0, hi], hi < 2^23,
-// c + 0.5 is exact and truncating it rounds halves up, which is roundf.
+// truncating c + 0.49999997 (the float below 0.5) gives roundf(c) for every
+// float c (checked exhaustively on the host); truncating c + 0.5 would turn
+// c = 0.49999997 into 1.  The clamp is one median-of-three instruction, which
+// for NaN returns the smaller of the two other operands, 0, like
+// fminf(fmaxf(v, 0), hi).
 __device__ __forceinline__ int32_t
 clamp_round(float v, float hi)
 {
-    return int32_t(fminf(fmaxf(v, 0.0f), hi) + 0.5f);
+    return int32_t(__builtin_amdgcn_fmed3f(v, 0.0f, hi) + 0.49999997f);
 }
 
 // Pixel window of one child for one camera (VH:236-247 and VH:365-380) and
@@ -457,30 +461,23 @@ template <bool PARTIAL>
 __device__ __forceinline__ void
 child_window(const Box& bb, const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1, ChildWindow& cw)
 {
-    int32_t x0, y0, x1, y1;
+    const float ex0 = bb.minx + -0.5f, ey0 = bb.miny + -0.5f;
+    const float ex1 = bb.maxx + 0.5f, ey1 = bb.maxy + 0.5f;
+    const int32_t x0 = clamp_round(ex0, Wm1), y0 = clamp_round(ey0, Hm1);
+    const int32_t x1 = clamp_round(ex1, Wm1), y1 = clamp_round(ey1, Hm1);
     bool same = true;
-    if constexpr (!PARTIAL)
+    if constexpr (PARTIAL)
     {
-        x0 = clamp_round(bb.minx + -0.5f, Wm1);
-        y0 = clamp_round(bb.miny + -0.5f, Hm1);
-        x1 = clamp_round(bb.maxx + 0.5f, Wm1);
-        y1 = clamp_round(bb.maxy + 0.5f, Hm1);
-    }
-    else
-    {
-        const float rx0 = roundf(bb.minx + -0.5f);
-        const float ry0 = roundf(bb.miny + -0.5f);
-        const float rx1 = roundf(fmaxf(bb.maxx, -FLT_MAX) + 0.5f);
-        const float ry1 = roundf(fmaxf(bb.maxy, -FLT_MAX) + 0.5f);
-        const float fx0 = fminf(fmaxf(rx0, 0.0f), Wm1), fy0 = fminf(fmaxf(ry0, 0.0f), Hm1);
-        const float fx1 = fminf(fmaxf(rx1, 0.0f), Wm1), fy1 = fminf(fmaxf(ry1, 0.0f), Hm1);
-        x0 = int32_t(fx0);
-        y0 = int32_t(fy0);
-        x1 = int32_t(fx1);
-        y1 = int32_t(fy1);
-        if (!(rx0 == fx0 && rx1 == fx1 && ry0 == fy0 && ry1 == fy1))
+        // roundf(e) lies in [0, n - 1] iff -0.5 < e < n - 0.5.  The box minimum never exceeds its
+        // maximum unless every corner is NaN (minimum FLT_MAX, maximum NaN: e1 fails its test), so
+        // two tests per axis tell whether the clamp changed any of the four rounded bounds.
+        if (!(ex0 > -0.5f && ey0 > -0.5f && ex1 < Wm1 + 0.5f && ey1 < Hm1 + 0.5f))
         {
-            // clamped somewhere: compare with the unclamped box area in int64 (wrap-around like the GPU reference)
+            // clamped somewhere: compare with the unclamped box area in int64 (wrap-around like the
+            // GPU reference); its maxima start at -FLT_MAX, which an all-NaN maximum keeps
+            const float rx0 = roundf(ex0), ry0 = roundf(ey0);
+            const float rx1 = roundf(fmaxf(bb.maxx, -FLT_MAX) + 0.5f);
+            const float ry1 = roundf(fmaxf(bb.maxy, -FLT_MAX) + 0.5f);
             const uint64_t fy = uint64_t(f2i64_sat(ry1)) - uint64_t(f2i64_sat(ry0)) + 1u;
             const uint64_t fx = uint64_t(f2i64_sat(rx1)) - uint64_t(f2i64_sat(rx0)) + 1u;
             same = int64_t(fy * fx) == int64_t(y1 - y0 + 1) * int64_t(x1 - x0 + 1);
@@ -491,11 +488,11 @@ child_window(const Box& bb, const float* __restrict__ I, uint32_t ld, float Hm1,
     // 32-bit byte offsets from the camera's plane (a plane holds fewer than 2^30 floats), so the reads
     // take the scalar-base + vector-offset form when the plane base is wave-uniform
     const auto G = (gptr<const char>)I;
-    const uint32_t r0 = uint32_t(y0) * ld, r1 = uint32_t(y1 + 1) * ld;
-    cw.i11 = *(global_float*)(G + (r1 + uint32_t(x1) + 1u) * 4u);
+    const uint32_t r0 = __umul24(uint32_t(y0), ld), r1 = __umul24(uint32_t(y1 + 1), ld); // y, ld < 2^24
+    cw.i11 = *(global_float*)(G + 4 + (r1 + uint32_t(x1)) * 4u);
     cw.i00 = *(global_float*)(G + (r0 + uint32_t(x0)) * 4u);
     cw.i10 = *(global_float*)(G + (r1 + uint32_t(x0)) * 4u);
-    cw.i01 = *(global_float*)(G + (r0 + uint32_t(x1) + 1u) * 4u);
+    cw.i01 = *(global_float*)(G + 4 + (r0 + uint32_t(x1)) * 4u);
     cw.area = float(y1 - y0 + 1) * float(x1 - x0 + 1);
 }
 
@@ -528,54 +525,103 @@ child_decide(const ChildWindow& cw, bool fully_inside, uint32_t bit, uint32_t& r
     }
 }
 
+// Largest magnitude of three values; NaN if any of them is NaN.
+__device__ __forceinline__ float
+max_abs3_nan(float a, float b, float c)
+{
+    return __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(a), fabsf(b)), fabsf(c));
+}
+
+// Carving divides only to take the pixel coordinate q + 1 (OpenGL) or
+// q - 0.5 (OpenCV) of a quotient q.  With every operand magnitude at most
+// 2^40 (hi, NaN failing) and the denominator's at least 2^-40 (lo_d), the
+// shared-reciprocal division gives those pixel coordinates exactly: a
+// numerator of at least 2^-80 keeps the IEEE division's scaling and fix-up
+// idle (see div2_shared), and a smaller one, zero included, gives |q| < 2^-40
+// on both routes, which leaves q + 1 at 1 and q - 0.5 at -0.5.
+__device__ __forceinline__ bool
+carve_div_in_range(float hi, float lo_d)
+{
+    return hi <= 0x1p40f && lo_d >= 0x1p-40f;
+}
+
+// Pixel coordinate of a projected coordinate q (VH:163-170): OpenGL maps
+// normalized device coordinates with image_utils.h:67-71 (t * 0.5 is exact, so
+// fmaf(t, 0.5, -0.5) == (t / 2) - 0.5), OpenCV subtracts the pixel centre.
+template <bool GL>
+__device__ __forceinline__ float
+to_pixel(float q, float size)
+{
+    if constexpr (GL)
+        return fmaf((q + 1.0f) * size, 0.5f, -0.5f);
+    else
+        return q - 0.5f;
+}
+
 // Projects one z-slice (3 x 3 points) of a parent's lattice for one camera and
 // reduces it to the pixel windows of its four 2 x 2 point groups, window a + 2b
-// for x offset a and y offset b; `inb` gets bit u + 3v for each point inside
-// the image with margin 1 (partial masks).  The fused row products are shared
-// along the lattice axes: fmaf(A2, z, A3) per slice, fmaf(A1, y, .) per row
-// and fmaf(A0, x, .) per point, the nested fmaf chain of VH:150-153.
-template <bool PARTIAL>
+// for x offset a and y offset b.  The fused row products are shared along the
+// lattice axes: fmaf(A2, z, A3) per slice, fmaf(A1, y, .) per row and
+// fmaf(A0, x, .) per point, the nested fmaf chain of VH:150-153.  The
+// quotients take the shared-reciprocal division, checked once for the whole
+// slice (carve_div_in_range; the largest magnitude propagates NaN, so a NaN
+// operand fails the check too); a slice that fails is projected again with
+// the IEEE division (rare: a lattice point on a camera's principal plane, or
+// non-finite input), and `nanpts` gets bit u + 3v for each point whose pixel
+// coordinates hold a NaN.  A slice that passes has finite pixel coordinates only.
+template <bool GL>
 __device__ __forceinline__ void
-lattice_slice(const CarveCamera& cam, bool gl, const float (&wx)[3], const float (&wy)[3], float wz, float fW, float fH,
-              float Wlim, float Hlim, Box (&win)[4], uint32_t& inb)
+lattice_slice(const CarveCamera& cam, const float (&wx)[3], const float (&wy)[3], float wz, float fW, float fH,
+              Box (&win)[4], uint32_t& nanpts)
 {
     const float ax = fmaf(cam.x2, wz, cam.x3);
     const float ay = fmaf(cam.y2, wz, cam.y3);
     const float ad = fmaf(cam.d2, wz, cam.d3);
     float ppx[3][3], ppy[3][3]; // [v][u]
-    inb = 0u;
+    float hi[3], lo[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v)
     {
         const float bx = fmaf(cam.x1, wy[v], ax);
         const float by = fmaf(cam.y1, wy[v], ay);
         const float bd = fmaf(cam.d1, wy[v], ad);
+        float h[3], l[3];
 #pragma unroll
         for (int u = 0; u < 3; ++u)
         {
             const float cx = fmaf(cam.x0, wx[u], bx);
             const float cy = fmaf(cam.y0, wx[u], by);
             const float cd = fmaf(cam.d0, wx[u], bd);
+            h[u] = max_abs3_nan(cx, cy, cd);
+            l[u] = fabsf(cd);
             float qx, qy;
-            div2_shared(cx, cy, cd, qx, qy); // cx / cd, cy / cd
-            float pxl, pyl;
-            if (gl)
+            div2_unchecked(cx, cy, cd, qx, qy); // cx / cd, cy / cd when in range
+            ppx[v][u] = to_pixel<GL>(qx, fW);
+            ppy[v][u] = to_pixel<GL>(qy, fH);
+        }
+        hi[v] = __builtin_elementwise_maximum(__builtin_elementwise_maximum(h[0], h[1]), h[2]);
+        lo[v] = fminf(fminf(l[0], l[1]), l[2]);
+    }
+    nanpts = 0u;
+    const float hs = __builtin_elementwise_maximum(__builtin_elementwise_maximum(hi[0], hi[1]), hi[2]);
+    if (!carve_div_in_range(hs, fminf(fminf(lo[0], lo[1]), lo[2])))
+    {
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+        {
+            const float bx = fmaf(cam.x1, wy[v], ax);
+            const float by = fmaf(cam.y1, wy[v], ay);
+            const float bd = fmaf(cam.d1, wy[v], ad);
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
             {
-                // image_utils.h:67-71; t * 0.5 is exact, so fmaf(t, 0.5, -0.5) == (t / 2) - 0.5
-                pxl = fmaf((qx + 1.0f) * fW, 0.5f, -0.5f);
-                pyl = fmaf((qy + 1.0f) * fH, 0.5f, -0.5f);
-            }
-            else
-            {
-                pxl = qx - 0.5f;
-                pyl = qy - 0.5f;
-            }
-            ppx[v][u] = pxl;
-            ppy[v][u] = pyl;
-            if constexpr (PARTIAL)
-            {
-                if (pxl >= 0.5f && pxl < Wlim && pyl >= 0.5f && pyl < Hlim)
-                    inb |= 1u << (u + 3 * v);
+                const float cx = fmaf(cam.x0, wx[u], bx);
+                const float cy = fmaf(cam.y0, wx[u], by);
+                const float cd = fmaf(cam.d0, wx[u], bd);
+                ppx[v][u] = to_pixel<GL>(cx / cd, fW);
+                ppy[v][u] = to_pixel<GL>(cy / cd, fH);
+                if (ppx[v][u] != ppx[v][u] || ppy[v][u] != ppy[v][u])
+                    nanpts |= 1u << (u + 3 * v);
             }
         }
     }
@@ -597,18 +643,47 @@ lattice_slice(const CarveCamera& cam, bool gl, const float (&wx)[3], const float
 }
 
 // The four children of layer cz (children 4cz .. 4cz + 3) between slices cz
-// (`lo`) and cz + 1 (`hi`): their integral reads are issued before the first
-// is used.  Returns the children this camera finds empty.
-template <bool PARTIAL>
-__device__ __forceinline__ uint32_t
-child_layer(int cz, const Box (&lo)[4], const Box (&hi)[4], uint32_t ilo, uint32_t ihi, uint32_t alive,
-            const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1, uint32_t& refine, uint32_t& inside_any)
+// (`lo`) and cz + 1 (`hi`): their windows, with the integral reads only
+// issued.  A child is fully inside the frame (VH:358-362, needed for partial
+// masks at the last level only) when none of its 8 corners is NaN and its
+// pixel box satisfies 1 <= roundf(p) <= n - 2 on both axes, i.e.
+// 0.5 <= p < n - 1.5.
+struct LayerWindows
 {
     ChildWindow cw[4];
+    bool inside[4];
+};
+
+template <bool PARTIAL>
+__device__ __forceinline__ void
+layer_windows(int cz, const Box (&lo)[4], const Box (&hi)[4], uint32_t nanpts, uint32_t alive, bool last,
+              const float* __restrict__ I, uint32_t ld, float Hm1, float Wm1, float Hlim, float Wlim, LayerWindows& L)
+{
 #pragma unroll
     for (int q = 0; q < 4; ++q)
+    {
+        L.inside[q] = false;
         if (alive & (1u << (4 * cz + q)))
-            child_window<PARTIAL>(box_merge(lo[q], hi[q]), I, ld, Hm1, Wm1, cw[q]);
+        {
+            const Box bb = box_merge(lo[q], hi[q]);
+            child_window<PARTIAL>(bb, I, ld, Hm1, Wm1, L.cw[q]);
+            if (PARTIAL && last)
+            {
+                const int a = q & 1, b = q >> 1;
+                const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * b); // the group's points in a slice
+                L.inside[q] = (nanpts & (mm | (mm << 9))) == 0u && bb.minx >= 0.5f && bb.miny >= 0.5f &&
+                              bb.maxx < Wlim && bb.maxy < Hlim;
+            }
+        }
+    }
+}
+
+// Decisions of the live children of layer cz; returns the children this
+// camera finds empty.
+template <bool PARTIAL>
+__device__ __forceinline__ uint32_t
+layer_decide(int cz, const LayerWindows& L, uint32_t alive, uint32_t& refine, uint32_t& inside_any)
+{
     uint32_t dead = 0u;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -616,14 +691,7 @@ child_layer(int cz, const Box (&lo)[4], const Box (&hi)[4], uint32_t ilo, uint32
         const uint32_t bit = 1u << (4 * cz + q);
         if (!(alive & bit))
             continue;
-        bool inside = false;
-        if constexpr (PARTIAL)
-        {
-            const int a = q & 1, bb = q >> 1;
-            const uint32_t mm = ((3u << a) | (3u << (a + 3))) << (3 * bb);
-            inside = ((ilo & mm) == mm) && ((ihi & mm) == mm);
-        }
-        if (child_decide<PARTIAL>(cw[q], inside, bit, refine, inside_any))
+        if (child_decide<PARTIAL>(L.cw[q], L.inside[q], bit, refine, inside_any))
             dead |= bit;
     }
     return dead;
@@ -631,7 +699,7 @@ child_layer(int cz, const Box (&lo)[4], const Box (&hi)[4], uint32_t ilo, uint32
 
 // Projects the 3x3x3 lattice of one parent for one camera and classifies the
 // live children.  `dead` collects children found empty by this camera.
-template <bool PARTIAL>
+template <bool PARTIAL, bool GL>
 __device__ __forceinline__ uint32_t
 parent_camera(const SceneDev& s,
               const float (&wx)[3],
@@ -639,29 +707,29 @@ parent_camera(const SceneDev& s,
               const float (&wz)[3],
               int32_t b,
               uint32_t alive,
+              bool last,
               uint32_t& refine,
               uint32_t& inside_any)
 {
-    const bool gl = s.opengl != 0;
     const float fH = float(s.H), fW = float(s.W);
     const float Hm1 = float(s.H - 1), Wm1 = float(s.W - 1);
-    // VH:358-362 with roundf folded into the comparison: 1 <= roundf(p) <= n-2  <=>  0.5 <= p < n-1.5
     const float Wlim = float(s.W) - 1.5f, Hlim = float(s.H) - 1.5f;
     const uint32_t ld = uint32_t(integral_pitch(s.W));
-    const CarveCamera cam = load_camera(s.transforms + 16 * b, gl);
+    const CarveCamera cam = load_camera(s.transforms + 16 * b, GL);
     // entry (y, x), y and x from -1, at I[(y + 1) * ld + x + 1]
     const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld + 3;
 
-    // one z-slice of the lattice at a time: the child boxes of layer cz need slices cz and cz + 1,
-    // so only two slices of windows are live (fewer registers, more waves in flight)
+    // one z-slice of the lattice at a time (only two slices of windows live); the reads of the lower
+    // layer are in flight while the top slice is projected and the upper layer's reads are issued
     Box w0[4], w1[4];
-    uint32_t i0 = 0u, i1 = 0u;
-    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[0], fW, fH, Wlim, Hlim, w0, i0);
-    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[1], fW, fH, Wlim, Hlim, w1, i1);
-    uint32_t dead = child_layer<PARTIAL>(0, w0, w1, i0, i1, alive, I, ld, Hm1, Wm1, refine, inside_any);
-    lattice_slice<PARTIAL>(cam, gl, wx, wy, wz[2], fW, fH, Wlim, Hlim, w0, i0);
-    dead |= child_layer<PARTIAL>(1, w1, w0, i1, i0, alive, I, ld, Hm1, Wm1, refine, inside_any);
-    return dead;
+    uint32_t n0 = 0u, n1 = 0u;
+    lattice_slice<GL>(cam, wx, wy, wz[0], fW, fH, w0, n0);
+    lattice_slice<GL>(cam, wx, wy, wz[1], fW, fH, w1, n1);
+    LayerWindows L0, L1;
+    layer_windows<PARTIAL>(0, w0, w1, n0 | (n1 << 9), alive, last, I, ld, Hm1, Wm1, Hlim, Wlim, L0);
+    lattice_slice<GL>(cam, wx, wy, wz[2], fW, fH, w0, n0);
+    layer_windows<PARTIAL>(1, w1, w0, n1 | (n0 << 9), alive, last, I, ld, Hm1, Wm1, Hlim, Wlim, L1);
+    return layer_decide<PARTIAL>(0, L0, alive, refine, inside_any) | layer_decide<PARTIAL>(1, L1, alive, refine, inside_any);
 }
 
 __device__ __forceinline__ void
@@ -679,7 +747,7 @@ lattice_world(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_t 
 }
 
 // All cameras in index order for one parent; returns the occupancy mask.
-template <bool PARTIAL>
+template <bool PARTIAL, bool GL, bool UNIFORM>
 __device__ uint32_t
 classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_t li, bool last, uint32_t& tests)
 {
@@ -691,7 +759,7 @@ classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_
         if (alive == 0u)
             break;
         tests += __popc(alive);
-        alive &= ~parent_camera<PARTIAL>(s, wx, wy, wz, b, alive, refine, inside_any);
+        alive &= ~parent_camera<PARTIAL, GL>(s, wx, wy, wz, b, alive, last, refine, inside_any);
     }
     uint32_t occ = refine & alive; // should_refine && !is_empty
     if (PARTIAL && last)
@@ -699,13 +767,28 @@ classify_parent(const SceneDev& s, uint32_t px, uint32_t py, uint32_t pz, int32_
     return occ;
 }
 
-template <bool PARTIAL>
+template <bool PARTIAL, bool GL, bool UNIFORM>
 __device__ __forceinline__ uint32_t
 classify_entry(const SceneDev& s, uint64_t key, int32_t level_i, uint32_t& tests)
 {
     if (level_i >= s.level)
         return 0u;
-    return classify_parent<PARTIAL>(s, key_x(key), key_y(key), key_z(key), level_i, level_i == s.level - 1, tests);
+    return classify_parent<PARTIAL, GL, UNIFORM>(s, key_x(key), key_y(key), key_z(key), level_i,
+                                                 level_i == s.level - 1, tests);
+}
+
+// UNIFORM = true for a descriptor shared by the whole wave.  The flag changes
+// no arithmetic; it keeps the wave-uniform and the per-lane instantiations
+// apart, so that the compiler cannot merge the two calls into one per-lane copy.
+template <bool UNIFORM>
+__device__ __forceinline__ uint32_t
+classify_scene(const SceneDev& s, uint64_t key, int32_t level_i, uint32_t& tests)
+{
+    if (s.partial)
+        return s.opengl ? classify_entry<true, true, UNIFORM>(s, key, level_i, tests)
+                        : classify_entry<true, false, UNIFORM>(s, key, level_i, tests);
+    return s.opengl ? classify_entry<false, true, UNIFORM>(s, key, level_i, tests)
+                    : classify_entry<false, false, UNIFORM>(s, key, level_i, tests);
 }
 
 __global__ __launch_bounds__(kCarveThreads) __attribute__((amdgpu_waves_per_eu(4))) void
@@ -725,23 +808,10 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
         const uint64_t key = parents[t];
         const uint32_t sid = key_scene(key);
         const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
-        if (__all(sid == s0))
-        {
-            // the whole wave works on one scene: camera data and integral bases are wave-uniform
-            const SceneDev s = scenes[s0];
-            if (s.partial)
-                mask = classify_entry<true>(s, key, level_i, tests);
-            else
-                mask = classify_entry<false>(s, key, level_i, tests);
-        }
+        if (__all(sid == s0)) // the whole wave carves one scene: camera data and integral bases are scalar
+            mask = classify_scene<true>(wave_uniform(scenes[s0]), key, level_i, tests);
         else
-        {
-            const SceneDev s = scenes[sid];
-            if (s.partial)
-                mask = classify_entry<true>(s, key, level_i, tests);
-            else
-                mask = classify_entry<false>(s, key, level_i, tests);
-        }
+            mask = classify_scene<false>(scenes[sid], key, level_i, tests);
         child_mask[t] = uint8_t(mask);
     }
     uint32_t total;
@@ -760,22 +830,52 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
 // child is empty if any camera says so, so the lanes combine exactly by
 // OR / min across the group.  The canonical early-exit count is rebuilt from
 // the first camera that found each child empty.
-template <int G, bool PARTIAL>
+template <int G, bool PARTIAL, bool GL, bool UNIFORM>
 __device__ __forceinline__ void
 classify_split(const SceneDev& s, uint64_t key, int32_t level_i, int32_t j, uint32_t& refine, uint32_t& inside,
                uint32_t (&first_empty)[8])
 {
     float wx[3], wy[3], wz[3];
     lattice_world(s, key_x(key), key_y(key), key_z(key), level_i, wx, wy, wz);
+    const bool last = level_i == s.level - 1;
     uint32_t alive = 0xffu;
     for (int32_t b = j; b < s.B && alive != 0u; b += G)
     {
-        const uint32_t dead = parent_camera<PARTIAL>(s, wx, wy, wz, b, alive, refine, inside);
+        const uint32_t dead = parent_camera<PARTIAL, GL>(s, wx, wy, wz, b, alive, last, refine, inside);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (dead & (1u << c))
                 first_empty[c] = uint32_t(b);
         alive &= ~dead;
+    }
+}
+
+// Scene flags and the classification of one parent by lane j of its group;
+// UNIFORM as for classify_scene.
+template <int G, bool UNIFORM>
+__device__ __forceinline__ void
+split_scene(const SceneDev& s, uint64_t key, int32_t level_i, int32_t j, int32_t& B, bool& partial, bool& last,
+            bool& active, uint32_t& refine, uint32_t& inside, uint32_t (&first_empty)[8])
+{
+    B = s.B;
+    partial = s.partial != 0;
+    last = level_i == s.level - 1;
+    active = level_i < s.level;
+    if (!active)
+        return;
+    if (partial)
+    {
+        if (s.opengl)
+            classify_split<G, true, true, UNIFORM>(s, key, level_i, j, refine, inside, first_empty);
+        else
+            classify_split<G, true, false, UNIFORM>(s, key, level_i, j, refine, inside, first_empty);
+    }
+    else
+    {
+        if (s.opengl)
+            classify_split<G, false, true, UNIFORM>(s, key, level_i, j, refine, inside, first_empty);
+        else
+            classify_split<G, false, false, UNIFORM>(s, key, level_i, j, refine, inside, first_empty);
     }
 }
 
@@ -802,18 +902,14 @@ carve_classify_split_kernel(const uint64_t* __restrict__ parents,
     if (p < n)
     {
         const uint64_t key = parents[p];
-        const SceneDev s = scenes[key_scene(key)];
-        B = s.B;
-        partial = s.partial != 0;
-        last = level_i == s.level - 1;
-        active = level_i < s.level;
-        if (active)
-        {
-            if (partial)
-                classify_split<G, true>(s, key, level_i, j, refine, inside, first_empty);
-            else
-                classify_split<G, false>(s, key, level_i, j, refine, inside, first_empty);
-        }
+        const uint32_t sid = key_scene(key);
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(sid);
+        if (__all(sid == s0))
+            split_scene<G, true>(wave_uniform(scenes[s0]), key, level_i, j, B, partial, last, active, refine, inside,
+                                 first_empty);
+        else
+            split_scene<G, false>(scenes[sid], key, level_i, j, B, partial, last, active, refine, inside,
+                                  first_empty);
     }
 #pragma unroll
     for (int off = G / 2; off >= 1; off >>= 1)
@@ -1203,8 +1299,9 @@ launch_prefix_extract(const uint32_t* bitmap,
 }
 
 // ===========================================================================
-// Self-test of div2_shared against the IEEE division on random operands in
-// (and at the edges of) its fast range and beyond it.
+// Self-test of the division shortcuts against the IEEE division on random
+// operands in (and at the edges of) their fast ranges and beyond them, and of
+// clamp_round against clamp(roundf(v)) on rounding boundaries and special values.
 // ===========================================================================
 
 __device__ __forceinline__ uint32_t
@@ -1228,6 +1325,29 @@ test_operand(uint64_t key)
     return __uint_as_float(((r >> 31) << 31) | (uint32_t(e + 127) << 23) | mant);
 }
 
+// A numerator for the carving check: one in four is tiny (2^-149 .. 2^-60,
+// subnormals included) or zero, the others as test_operand.
+__device__ __forceinline__ float
+test_numerator(uint64_t key)
+{
+    const uint32_t r = mix32(key ^ 0x5851f42d4c957f2dull);
+    if ((r & 3u) != 0u)
+        return test_operand(key);
+    const uint32_t m = mix32(key ^ 0x14057b7ef767814full);
+    const uint32_t bits = ((r >> 2) % 90u == 0u) ? 0u : (m % 0x21800000u); // 0x21800000 = 2^-60
+    return __uint_as_float(((r >> 31) << 31) | bits);
+}
+
+// The reference's window bound: clamp(int64(roundf(v)), 0, hi).
+__device__ __forceinline__ int32_t
+clamp_round_ref(float v, float hi)
+{
+    const float r = roundf(v);
+    if (r != r)
+        return 0;
+    return int32_t(fminf(fmaxf(r, 0.0f), hi));
+}
+
 __global__ __launch_bounds__(256) void
 div2_check_kernel(uint64_t count, uint64_t seed, unsigned long long* mismatches)
 {
@@ -1241,6 +1361,40 @@ div2_check_kernel(uint64_t count, uint64_t seed, unsigned long long* mismatches)
         div2_shared(nx, ny, d, qx, qy);
         const float rx = nx / d, ry = ny / d;
         bad = (__float_as_uint(qx) != __float_as_uint(rx) ? 1u : 0u) + (__float_as_uint(qy) != __float_as_uint(ry) ? 1u : 0u);
+
+        // carving: operands with a carve_div_in_range denominator give the pixel coordinates of the
+        // IEEE quotients in both conventions, tiny and zero numerators included
+        const float tx = test_numerator(k), ty = test_numerator(k + 1);
+        if (carve_div_in_range(max_abs3_nan(tx, ty, d), fabsf(d)))
+        {
+            float cx, cy;
+            div2_unchecked(tx, ty, d, cx, cy);
+            const float ex = tx / d, ey = ty / d;
+            const float fW = float(1u + mix32(k ^ 7u) % 4096u);
+            bad += __float_as_uint(to_pixel<true>(cx, fW)) != __float_as_uint(to_pixel<true>(ex, fW)) ? 1u : 0u;
+            bad += __float_as_uint(to_pixel<true>(cy, fW)) != __float_as_uint(to_pixel<true>(ey, fW)) ? 1u : 0u;
+            bad += __float_as_uint(to_pixel<false>(cx, fW)) != __float_as_uint(to_pixel<false>(ex, fW)) ? 1u : 0u;
+            bad += __float_as_uint(to_pixel<false>(cy, fW)) != __float_as_uint(to_pixel<false>(ey, fW)) ? 1u : 0u;
+        }
+
+        // clamp_round on quarter steps around the rounding boundaries, one ulp either side, and
+        // specials (NaN, infinities, signed zeros, huge values)
+        const uint32_t c = mix32(k ^ 0xabcdefull);
+        const float hi = float(c % 3000u);
+        float v = float(int32_t(c >> 12) % 8200 - 4100) * 0.25f;
+        if ((c & 0x30u) != 0u && v != 0.0f) // one ulp away from zero or towards it
+            v = __uint_as_float(__float_as_uint(v) + ((c & 0x30u) == 0x10u ? 1u : 0xffffffffu));
+        switch (c & 0xf00u)
+        {
+        case 0x100u: v = __uint_as_float(0x7fc00000u); break;
+        case 0x200u: v = INFINITY; break;
+        case 0x300u: v = -INFINITY; break;
+        case 0x400u: v = -0.0f; break;
+        case 0x500u: v = 3.0e38f; break;
+        case 0x600u: v = -3.0e38f; break;
+        default: break;
+        }
+        bad += clamp_round(v, hi) != clamp_round_ref(v, hi) ? 1u : 0u;
     }
     bad = __reduce_add_sync(0xffffffffffffffffull, bad);
     if ((threadIdx.x & 63) == 0 && bad != 0u)
@@ -1363,7 +1517,8 @@ field_factor(const FieldSample<T>& f, float pad)
 
 // Product over the cameras, in camera order, of the bilinear mask samples at
 // a grid vertex (VH:470-530); masks of element type T (float32 or uint8).
-template <typename T>
+// UNIFORM as for classify_scene: keeps the wave-uniform and per-lane copies apart.
+template <typename T, bool UNIFORM>
 __device__ __forceinline__ float
 field_value(const SceneDev& s, uint64_t key)
 {
@@ -1409,13 +1564,13 @@ field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* 
     float value;
     if (__all(sid == s0))
     {
-        const SceneDev s = scenes[s0]; // wave-uniform: scalar camera data
-        value = s.mask_dtype == kF32 ? field_value<float>(s, key) : field_value<uint8_t>(s, key);
+        const SceneDev s = wave_uniform(scenes[s0]); // scalar camera data
+        value = s.mask_dtype == kF32 ? field_value<float, true>(s, key) : field_value<uint8_t, true>(s, key);
     }
     else
     {
         const SceneDev s = scenes[sid];
-        value = s.mask_dtype == kF32 ? field_value<float>(s, key) : field_value<uint8_t>(s, key);
+        value = s.mask_dtype == kF32 ? field_value<float, false>(s, key) : field_value<uint8_t, false>(s, key);
     }
     values[i] = value;
 }
