@@ -272,7 +272,8 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
                  int32_t ts_rt,
                  bool sparse,
                  const To* __restrict__ weights,
-                 To norm)
+                 To norm,
+                 bool vec4)
 {
     extern __shared__ unsigned char smem_raw[];
     To* tile = reinterpret_cast<To*>(smem_raw);
@@ -341,6 +342,28 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     }
     if (!__syncthreads_or(band != 0u))
     {
+        if constexpr (std::is_same_v<Ti, float> && std::is_same_v<To, float>)
+        {
+            if (vec4)
+            {
+                // float32 rows of a multiple of 4 pixels, 16-byte aligned: thread t copies 4 columns
+                // (x0 + 4 (t % 16) ...) of rows y0 + t / 16 + 16 r with 16-byte loads and stores
+                constexpr int R4 = RB / 16;
+                const int32_t cx = x0 + 4 * int32_t(threadIdx.x & 15), ry = y0 + int32_t(threadIdx.x >> 4);
+                if (cx >= W)
+                    return;
+                float4_v v[R4];
+#pragma unroll
+                for (int r = 0; r < R4; ++r)
+                    if (ry + 16 * r < H)
+                        v[r] = *(gptr<const float4_v>)(I + int64_t(ry + 16 * r) * W + cx);
+#pragma unroll
+                for (int r = 0; r < R4; ++r)
+                    if (ry + 16 * r < H)
+                        *(gptr<float4_v>)(O + int64_t(ry + 16 * r) * W + cx) = v[r];
+                return;
+            }
+        }
         To v[P];
 #pragma unroll
         for (int q = 0; q < P; ++q)
@@ -498,23 +521,29 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, boo
     const int32_t n = int32_t(jobs.size());
     const int32_t h = k / 2;
     const size_t lds = size_t(kBlurBX + 2 * h) * (RB + 2 * h) * sizeof(To);
+    // blocks without band pixels copy with 16-byte accesses when every image allows it
+    bool vec4 = std::is_same_v<Ti, float> && std::is_same_v<To, float>;
+    for (const auto& j : jobs)
+        vec4 = vec4 && j.W % 4 == 0 && reinterpret_cast<uintptr_t>(j.in) % 16 == 0 &&
+               reinterpret_cast<uintptr_t>(j.out) % 16 == 0;
     const int32_t tsl = ts > 0 ? ts : 1;
     // float outputs of the common odd sizes take a kernel with the taps unrolled
     constexpr bool spec = std::is_same_v<To, float> && (std::is_same_v<Ti, float> || std::is_same_v<Ti, uint8_t>);
     if (spec && k == 3)
         blur_tile_kernel<Ti, To, RB, spec ? 3 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
-                                                                                   norm);
+                                                                                   norm, vec4);
     else if (spec && k == 5)
         blur_tile_kernel<Ti, To, RB, spec ? 5 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
-                                                                                   norm);
+                                                                                   norm, vec4);
     else if (spec && k == 7)
         blur_tile_kernel<Ti, To, RB, spec ? 7 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
-                                                                                   norm);
+                                                                                   norm, vec4);
     else if (spec && k == 9)
         blur_tile_kernel<Ti, To, RB, spec ? 9 : 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights,
-                                                                                   norm);
+                                                                                   norm, vec4);
     else
-        blur_tile_kernel<Ti, To, RB, 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights, norm);
+        blur_tile_kernel<Ti, To, RB, 0><<<unsigned(pb), 256, lds, st>>>(djobs, n, k, tsl, sparse, weights, norm,
+                                                                         vec4);
     THULL_LAUNCH_CHECK();
 }
 
