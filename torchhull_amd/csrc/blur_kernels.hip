@@ -280,7 +280,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     constexpr int P = RB / 4; // pixels per thread
     const int32_t k = KT > 0 ? KT : k_rt;
     const int32_t ts = KT > 0 ? (KT / 2 > 0 ? KT / 2 : 1) : ts_rt; // compile-time divisor when KT is fixed
-    const uint64_t blk = blockIdx.x;
+    const uint64_t blk = xcd_block(blockIdx.x, gridDim.x); // halos of neighbouring blocks share one L2
     const BlurJob j = jobs[find_blur_job(jobs, njobs, blk)]; // by value: no reloads after stores
     const uint32_t local = uint32_t(blk - j.p_begin);
     const uint32_t per = j.bx * j.by;

@@ -339,6 +339,24 @@ wave_uniform(const SceneDev& s)
     return u;
 }
 
+// Logical block of hardware workgroup `bid` of a 1-D grid of `nblocks`.  The
+// dispatcher hands workgroups to the 8 XCDs round-robin (workgroup i to XCD
+// i % 8), and every XCD has its own L2.  This bijection gives each XCD runs of
+// C consecutive logical blocks (neighbouring cells, vertices or pixels, which
+// read neighbouring data) while consecutive runs still rotate over the XCDs,
+// so work that varies along the grid (scenes with more cameras) stays spread
+// evenly.  Workgroups past the last whole round of 8 C keep their index.
+template <uint32_t C = 16>
+__device__ __forceinline__ uint32_t
+xcd_block(uint32_t bid, uint32_t nblocks)
+{
+    constexpr uint32_t kXcds = 8;
+    if (bid >= nblocks / (kXcds * C) * (kXcds * C))
+        return bid;
+    const uint32_t x = bid % kXcds, k = bid / kXcds;
+    return ((k / C) * kXcds + x) * C + k % C;
+}
+
 // Quotient of a 64-bit value by a 32-bit divisor, through the 32-bit divide
 // when the dividend fits (the common case; 64-bit division is a long
 // software sequence).

@@ -201,7 +201,7 @@ template <typename Lookup, typename V>
 __global__ __launch_bounds__(256) void
 mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* __restrict__ ntv)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     if (i >= p.nverts)
         return;
     const VertexRef r = vertex_ref(p, i);
@@ -278,7 +278,7 @@ template <typename Lookup>
 __global__ __launch_bounds__(256) void
 mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t* __restrict__ emask)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     if (i >= p.nverts)
         return;
     const uint32_t inf = info[i];
@@ -322,7 +322,7 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
                    float* __restrict__ verts)
 {
     __shared__ float stage[kMcWaves][64 * 3 * 3];
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool valid = i < p.nverts;
     const uint32_t m = valid ? emask[i] : 0u;
@@ -388,7 +388,7 @@ mc_faces_kernel(McPlan p,
                 float* __restrict__ soup_verts,
                 int64_t* __restrict__ faces)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     if (i >= p.nverts)
         return;
     const uint32_t inf = info[i];
@@ -449,7 +449,7 @@ mc_faces_unique_kernel(McPlan p,
                        const uint32_t* __restrict__ scene_vstart,
                        int64_t* __restrict__ faces)
 {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     if (i >= p.nverts)
         return;
     const uint32_t inf = info[i];

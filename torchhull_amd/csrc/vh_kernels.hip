@@ -801,7 +801,8 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
                       unsigned long long* __restrict__ tests_out)
 {
     __shared__ uint32_t lds[kCarveThreads / 64];
-    const uint32_t t = blockIdx.x * kCarveThreads + threadIdx.x;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x); // neighbouring parents read neighbouring pixels
+    const uint32_t t = lb * kCarveThreads + threadIdx.x;
     uint32_t mask = 0u, tests = 0u;
     if (t < n)
     {
@@ -817,7 +818,7 @@ carve_classify_kernel(const uint64_t* __restrict__ parents,
     uint32_t total;
     block_exclusive_sum(__popc(mask), lds, total);
     if (threadIdx.x == 0)
-        block_counts[blockIdx.x] = total;
+        block_counts[lb] = total;
     uint32_t tsum;
     block_exclusive_sum(tests, lds, tsum);
     if (threadIdx.x == 0 && tests_out != nullptr)
@@ -889,7 +890,7 @@ carve_classify_split_kernel(const uint64_t* __restrict__ parents,
                             unsigned long long* __restrict__ tests_out)
 {
     __shared__ uint32_t lds[256 / 64];
-    const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t gt = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const uint32_t p = gt / G;
     const int32_t j = int32_t(gt % G);
     uint32_t refine = 0u, inside = 0u;
@@ -1555,7 +1556,7 @@ __global__ __launch_bounds__(256) void
 field_kernel(const uint64_t* __restrict__ vertices, uint32_t n, const SceneDev* __restrict__ scenes,
              float* __restrict__ values)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (i >= n)
         return;
     const uint64_t key = vertices[i];
