@@ -959,25 +959,67 @@ carve_count_kernel(const uint8_t* __restrict__ child_mask, uint32_t n, uint32_t*
 }
 
 // The kept children of a parent whose scene ends at child_level are leaf
-// cells: set their bits in the scene's leaf-cell bitmap (rows of
-// cell_row_words words; the 2 x 2 x 2 children touch at most 4 words).
+// cells: their bits go into the scene's leaf-cell bitmap (rows of
+// cell_row_words words; the 2 x 2 x 2 children touch at most 4 words, one per
+// (y, z) row).  Parents whose children share a word (same scene, z, y and
+// x / 16) often sit in consecutive lanes (x-neighbours among the children of
+// one grandparent are listed together): the wave ORs the words of such runs
+// together (a segmented reduction over runs of up to 16 lanes; a key that
+// recurs further on just starts another run) and only the first lane of each
+// run issues the atomics.  Every lane of the wave calls this; `mask` is 0 for
+// lanes without children.
 __device__ __forceinline__ void
-mark_leaf_children(const SceneDev& sd, int32_t child_level, uint32_t cx, uint32_t cy, uint32_t cz, uint32_t m,
-                   uint32_t* __restrict__ cells)
+mark_leaf_children_wave(const SceneDev* __restrict__ scenes, int32_t child_level, uint64_t key, uint32_t mask,
+                        uint32_t* __restrict__ cells)
 {
-    if (sd.level != child_level)
-        return;
-    const uint64_t R = uint64_t(1) << child_level;
-    uint32_t* base = cells + sd.cell_offset + (cx >> 5);
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
+    uint32_t w[4] = { 0u, 0u, 0u, 0u };
+    uint64_t row0 = 0, row_step = 0, plane_step = 0;
+    bool mark = false;
+    if (mask != 0u)
+    {
+        const SceneDev& sd = scenes[key_scene(key)];
+        if (sd.level == child_level)
         {
-            const uint32_t bits = (m >> (2 * b + 4 * c)) & 3u; // children (x = 0, 1; y = b; z = c)
-            if (bits != 0u)
-                atomicOr(base + (uint64_t(cz + c) * R + (cy + b)) * sd.cell_row_words, bits << (cx & 31u));
+            mark = true;
+            const uint32_t cx = 2 * key_x(key), cy = 2 * key_y(key), cz = 2 * key_z(key);
+            const uint64_t R = uint64_t(1) << child_level;
+            row_step = sd.cell_row_words;
+            plane_step = R * row_step;
+            row0 = sd.cell_offset + uint64_t(cz) * plane_step + uint64_t(cy) * row_step + (cx >> 5);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) // r = b + 2c: children (x = 0, 1; y = b; z = c)
+                w[r] = ((mask >> (2 * r)) & 3u) << (cx & 31u);
         }
+    }
+    // run key: the parent's key with the low 4 bits of x cleared (x / 16 picks the word)
+    const uint64_t K = mark ? (key & ~uint64_t(15)) : ~uint64_t(0);
+    const uint32_t klo = uint32_t(K), khi = uint32_t(K >> 32);
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1)
+    {
+        // every lane shuffles (no short-circuit around a cross-lane read); lanes past the end read
+        // their own values, which leaves them unchanged
+        const uint32_t olo = uint32_t(__shfl_down(int(klo), d, 64));
+        const uint32_t ohi = uint32_t(__shfl_down(int(khi), d, 64));
+        const bool same = (olo == klo) & (ohi == khi);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const uint32_t o = uint32_t(__shfl_down(int(w[r]), d, 64));
+            w[r] |= same ? o : 0u;
+        }
+    }
+    const int lane = int(threadIdx.x & 63);
+    const uint32_t plo = uint32_t(__shfl_up(int(klo), 1, 64));
+    const uint32_t phi = uint32_t(__shfl_up(int(khi), 1, 64));
+    const bool first = (lane == 0) | (plo != klo) | (phi != khi);
+    if (mark && first)
+    {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (w[r] != 0u)
+                atomicOr(cells + row0 + uint64_t(r >> 1) * plane_step + uint64_t(r & 1) * row_step, w[r]);
+    }
 }
 
 __global__ __launch_bounds__(kCarveThreads) void
@@ -995,17 +1037,19 @@ carve_scatter_kernel(const uint64_t* __restrict__ parents,
     const uint32_t mask = (t < n) ? child_mask[t] : 0u;
     uint32_t total;
     uint32_t off = block_offsets[blockIdx.x] + block_exclusive_sum(__popc(mask), lds, total);
-    if (mask == 0u)
-        return;
-    const uint64_t key = parents[t];
-    const uint32_t s = key_scene(key), x = 2 * key_x(key), y = 2 * key_y(key), z = 2 * key_z(key);
-    // VH:694-711: children of a parent in child order (x bit, y bit, z bit)
+    uint64_t key = 0;
+    if (mask != 0u)
+    {
+        key = parents[t];
+        const uint32_t s = key_scene(key), x = 2 * key_x(key), y = 2 * key_y(key), z = 2 * key_z(key);
+        // VH:694-711: children of a parent in child order (x bit, y bit, z bit)
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-        if (mask & (1u << c))
-            children[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
+        for (int c = 0; c < 8; ++c)
+            if (mask & (1u << c))
+                children[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
+    }
     if (cells != nullptr)
-        mark_leaf_children(scenes[s], child_level, x, y, z, mask, cells);
+        mark_leaf_children_wave(scenes, child_level, key, mask, cells);
 }
 
 int32_t
