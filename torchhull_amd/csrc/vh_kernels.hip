@@ -1153,41 +1153,81 @@ mark_root_cells_kernel(const SceneDev* __restrict__ scenes, int32_t nscenes, uin
 
 // Vertex bitmap = leaf cells dilated by {0,1}^3: vertex (x, y, z) is a corner
 // of cell (x - dx, y - dy, z - dz).  One thread per vertex word, no atomics.
-__device__ __forceinline__ uint32_t
-dilate_word(const SceneDev& s, uint64_t local, const uint32_t* __restrict__ cells)
+// The reads of a word (the words xw and xw - 1 of 4 cell rows) are issued
+// unconditionally, out-of-range ones from word 0 with their result masked, so
+// that the reads of several words are in flight together.
+struct DilateReads
 {
-    const uint64_t R = uint64_t(1) << s.level, R1 = R + 1;
+    uint32_t c[4] = { 0u, 0u, 0u, 0u }; // cell words xw of rows (dy, dz), r = dy + 2 dz (0 when out of range)
+    uint32_t p[4] = { 0u, 0u, 0u, 0u }; // words xw - 1 of the same rows, read by lane 0 of a wave only
+    bool first_in_row = true;           // xw == 0: no word xw - 1
+};
+
+// Position of a vertex word in its scene's bitmap: word xw of row y + z (R + 1)
+// (rows of row_words words, R + 1 rows per plane).  Walking a thread's words
+// (a fixed stride apart) moves the position by a fixed (rows, words) step, so
+// only the first word of a scene needs divisions.
+struct WordPos
+{
+    uint32_t xw, y, z;
+};
+
+__device__ __forceinline__ WordPos
+word_pos(const SceneDev& s, uint64_t local)
+{
+    const uint32_t R1 = (1u << s.level) + 1u;
     const uint64_t row = div_u64_u32(local, s.row_words);
-    const uint32_t xw = uint32_t(local - row * s.row_words);
-    const uint64_t zq = div_u64_u32(row, uint32_t(R1));
-    const int64_t y = int64_t(row - zq * R1), z = int64_t(zq);
+    WordPos q;
+    q.xw = uint32_t(local - row * s.row_words);
+    const uint64_t zq = div_u64_u32(row, R1);
+    q.y = uint32_t(row - zq * R1);
+    q.z = uint32_t(zq);
+    return q;
+}
+
+// Issues the reads of word q (UNIFORM as for classify_scene: keeps the
+// wave-uniform and per-lane copies apart).  The lanes of a wave hold
+// consecutive words, so word xw - 1 of a row is the previous lane's word xw:
+// only lane 0 reads it.
+template <bool UNIFORM>
+__device__ __forceinline__ void
+dilate_issue(const SceneDev& s, const WordPos& q, const uint32_t* __restrict__ cells, bool lane0, DilateReads& d)
+{
+    const int32_t R = int32_t(1) << s.level;
     const uint32_t rwc = s.cell_row_words;
+    const uint32_t* plane = cells + s.cell_offset; // a scene's cell bitmap holds fewer than 2^32 words
+    d.first_in_row = q.xw == 0u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+    {
+        const int32_t cz = int32_t(q.z) - (r >> 1), cy = int32_t(q.y) - (r & 1);
+        const bool rowok = cz >= 0 && cz < R && cy >= 0 && cy < R;
+        const uint32_t base = (uint32_t(cz) * uint32_t(R) + uint32_t(cy)) * rwc;
+        d.c[r] = (rowok && q.xw < rwc) ? plane[base + q.xw] : 0u;
+        if (lane0)
+            d.p[r] = (rowok && q.xw > 0 && q.xw - 1 < rwc) ? plane[base + q.xw - 1] : 0u;
+    }
+}
+
+// Every lane of the wave calls this (cross-lane reads).
+__device__ __forceinline__ uint32_t
+dilate_combine(const DilateReads& d, bool lane0)
+{
     uint32_t out = 0u;
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
+    for (int r = 0; r < 4; ++r)
     {
-        const int64_t cz = z - dz;
-        if (cz < 0 || cz >= int64_t(R))
-            continue;
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-        {
-            const int64_t cy = y - dy;
-            if (cy < 0 || cy >= int64_t(R))
-                continue;
-            const uint32_t* rowp = cells + s.cell_offset + (uint64_t(cz) * R + uint64_t(cy)) * rwc;
-            const uint32_t cw = xw < rwc ? rowp[xw] : 0u;
-            const uint32_t pw = (xw > 0 && xw - 1 < rwc) ? rowp[xw - 1] : 0u;
-            out |= cw | (cw << 1) | (pw >> 31);
-        }
+        const uint32_t prev = uint32_t(__shfl_up(int(d.c[r]), 1, 64));
+        const uint32_t pw = d.first_in_row ? 0u : (lane0 ? d.p[r] : prev);
+        out |= d.c[r] | (d.c[r] << 1) | (pw >> 31);
     }
     return out;
 }
 
 // Each block covers kScanTile consecutive words, each thread one word per
-// 256; the scene lookup is done once per wave and step.  The block also sums
-// the popcounts of its words: the first pass of the exclusive scan that
-// numbers the vertices (the tile is that of the scan's down-sweep).
+// 256, in batches of 4 words whose reads are in flight together.  The block
+// also sums the popcounts of its words: the first pass of the exclusive scan
+// that numbers the vertices (the tile is that of the scan's down-sweep).
 __global__ __launch_bounds__(kScanThreads) void
 dilate_cells_kernel(const uint32_t* __restrict__ cells,
                     uint32_t* __restrict__ verts,
@@ -1197,31 +1237,78 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
                     int32_t nscenes,
                     uint32_t* __restrict__ block_sums)
 {
+    constexpr int kBatch = 4;
     __shared__ uint32_t lds[kScanThreads / 64];
     const uint64_t w0 = uint64_t(blockIdx.x) * kScanTile + threadIdx.x;
+    const bool lane0 = (threadIdx.x & 63) == 0;
     const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
     // all lanes take part in the lookup ballot before any diverge
     int32_t lo = find_segment_rising(begin, nscenes, w0);
+    int32_t at = -1;           // scene of the position below
+    WordPos q{ 0u, 0u, 0u };   // position of the previous word of this thread in scene `at`
+    uint32_t step_w = 0u, step_r = 0u, R1 = 0u, rw = 0u;
     uint32_t count = 0u;
-    for (int k = 0; k < kScanItems; ++k)
+    for (int k0 = 0; k0 < kScanItems; k0 += kBatch)
     {
-        const uint64_t w = w0 + uint64_t(k) * kScanThreads;
-        while (lo + 1 < nscenes && begin(lo + 1) <= w)
-            ++lo;
-        if (w >= nwords)
-            break;
-        const uint64_t local = w - begin(lo);
-        const int32_t l0 = __builtin_amdgcn_readfirstlane(lo);
-        uint32_t out;
-        if (__all(lo == l0))
+        DilateReads d[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
         {
-            const SceneDev sd = scenes[l0]; // wave-uniform: scalar loads
-            out = dilate_word(sd, local, cells);
+            const uint64_t w = w0 + uint64_t(k0 + u) * kScanThreads;
+            if (w < nwords)
+            {
+                while (lo + 1 < nscenes && begin(lo + 1) <= w)
+                    ++lo;
+                const SceneDev& sd = scenes[lo];
+                if (lo != at)
+                {
+                    // first word of this thread in the scene: divisions, and the per-stride step
+                    q = word_pos(sd, w - begin(lo));
+                    at = lo;
+                    rw = sd.row_words;
+                    R1 = (1u << sd.level) + 1u;
+                    step_r = uint32_t(kScanThreads) / rw;
+                    step_w = uint32_t(kScanThreads) - step_r * rw;
+                }
+                else
+                {
+                    q.xw += step_w;
+                    uint32_t dr = step_r;
+                    if (q.xw >= rw)
+                    {
+                        q.xw -= rw;
+                        ++dr;
+                    }
+                    q.y += dr;
+                    if (dr < R1)
+                    {
+                        if (q.y >= R1)
+                        {
+                            q.y -= R1;
+                            ++q.z;
+                        }
+                    }
+                    else
+                        q = word_pos(sd, w - begin(lo)); // tiny grids: more rows per stride than per plane
+                }
+                const int32_t l0 = __builtin_amdgcn_readfirstlane(lo);
+                if (__all(lo == l0))
+                    dilate_issue<true>(wave_uniform(scenes[l0]), q, cells, lane0, d[u]);
+                else
+                    dilate_issue<false>(sd, q, cells, lane0, d[u]);
+            }
         }
-        else
-            out = dilate_word(scenes[lo], local, cells);
-        verts[w] = out;
-        count += __popc(out);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            const uint64_t w = w0 + uint64_t(k0 + u) * kScanThreads;
+            const uint32_t out = dilate_combine(d[u], lane0); // all lanes: cross-lane reads
+            if (w < nwords)
+            {
+                verts[w] = out;
+                count += __popc(out);
+            }
+        }
     }
     uint32_t total;
     block_exclusive_sum(count, lds, total);
