@@ -321,21 +321,22 @@ find_segment_rising(F begin, int32_t n, uint64_t v)
     return s;
 }
 
-// The scene descriptor with every 32-bit word read from the first active
-// lane: for a descriptor that all active lanes share, the compiler then keeps
-// it, and what it loads through its pointers, in scalar registers.
-__device__ __forceinline__ SceneDev
-wave_uniform(const SceneDev& s)
+// A descriptor (scene, grid) with every 32-bit word read from the first
+// active lane: for a descriptor that all active lanes share, the compiler then
+// keeps it, and what it loads through its pointers, in scalar registers.
+template <typename T>
+__device__ __forceinline__ T
+wave_uniform(const T& s)
 {
-    constexpr int kWords = sizeof(SceneDev) / 4;
-    static_assert(sizeof(SceneDev) % 4 == 0, "SceneDev is a whole number of words");
+    constexpr int kWords = sizeof(T) / 4;
+    static_assert(sizeof(T) % 4 == 0, "descriptor is a whole number of words");
     uint32_t w[kWords];
-    __builtin_memcpy(w, &s, sizeof(SceneDev));
+    __builtin_memcpy(w, &s, sizeof(T));
 #pragma unroll
     for (int i = 0; i < kWords; ++i)
         w[i] = __builtin_amdgcn_readfirstlane(w[i]);
-    SceneDev u;
-    __builtin_memcpy(&u, w, sizeof(SceneDev));
+    T u;
+    __builtin_memcpy(&u, w, sizeof(T));
     return u;
 }
 

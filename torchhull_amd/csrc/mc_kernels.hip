@@ -19,6 +19,8 @@
 // bit-identical (same endpoints and values), so computing each edge's position
 // once gives the same coordinates.
 
+#include <type_traits>
+
 #include "mc_cases.h"
 #include "kernels.h"
 
@@ -197,6 +199,170 @@ finish_vertex(const McGrid& g, const float v[3], float* out)
 // Pass 1: cell case, completeness and edge sign changes per vertex
 // ---------------------------------------------------------------------------
 
+// Presence and rank of the cell's 7 other corners, in case-corner order 1, 4,
+// 3 (the +x, +y, +z neighbours), then 2, 5, 6, 7.  found[k]: inside the grid
+// and present; pos[k] is set where `need` says so (rank reads only for those).
+// With the bitmap policy the two corners x and x + 1 of one (y, z) row share a
+// bitmap word (and its prefix): 3 row words plus the vertex's own word are
+// read instead of 7 words and 7 prefixes, and the +x neighbour of a present
+// vertex i is vertex i + 1.
+template <typename Lookup>
+__device__ __forceinline__ void
+cell_corners(const Lookup& lk, const McGrid& g, uint64_t i, const VertexRef& r, bool (&found)[7],
+             typename Lookup::Probe (&pr)[7], bool& inx, bool& iny, bool& inz)
+{
+    constexpr uint32_t corner[7] = { 1u, 4u, 3u, 2u, 5u, 6u, 7u };
+    inx = r.x + 1 < g.dims[0];
+    iny = r.y + 1 < g.dims[1];
+    inz = r.z + 1 < g.dims[2];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+    {
+        const uint32_t o = c_corner_off_host[corner[k]];
+        const bool in = (!(o & 1u) || inx) && (!((o >> 1) & 1u) || iny) && (!((o >> 2) & 1u) || inz);
+        if (in)
+            pr[k] = lk.probe(g, r.x + (o & 1u), r.y + ((o >> 1) & 1u), r.z + ((o >> 2) & 1u));
+        found[k] = in && lk.present(pr[k]);
+    }
+}
+
+// Bitmap policy: row words shared by the corners x and x + 1.
+struct RowProbe
+{
+    uint64_t w;     // word of x
+    uint32_t word;  // bitmap word of x
+    uint32_t next;  // bitmap word of x + 1 when x % 32 == 31 (else unused)
+    bool p0, p1;    // x, x + 1 present (x + 1 only when inside the grid)
+};
+
+__device__ __forceinline__ RowProbe
+row_probe(const BitmapLookup& lk, const McGrid& g, uint32_t x, uint32_t y, uint32_t z, bool in_row, bool inx)
+{
+    RowProbe q{};
+    if (!in_row)
+        return q;
+    q.w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
+    q.word = lk.bitmap[q.w];
+    const uint32_t b = x & 31u;
+    q.p0 = ((q.word >> b) & 1u) != 0u;
+    if (inx)
+    {
+        if (b != 31u)
+            q.p1 = ((q.word >> (b + 1u)) & 1u) != 0u;
+        else
+        {
+            q.next = lk.bitmap[q.w + 1];
+            q.p1 = (q.next & 1u) != 0u;
+        }
+    }
+    return q;
+}
+
+// rank of x (r0) and x + 1 (r1) in a row with at least one of them needed
+__device__ __forceinline__ void
+row_ranks(const BitmapLookup& lk, const RowProbe& q, uint32_t x, uint32_t& r0, uint32_t& r1)
+{
+    const uint32_t b = x & 31u;
+    r0 = lk.prefix[q.w] + __popc(q.word & ((1u << b) - 1u));
+    r1 = b != 31u ? r0 + (q.p0 ? 1u : 0u) : lk.prefix[q.w + 1];
+}
+
+template <typename Lookup, typename V, bool UNIFORM>
+__device__ __forceinline__ void
+mc_classify_vertex(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t i, const VertexRef& r, uint32_t& info,
+                   uint32_t& ntv)
+{
+    const V iso = static_cast<V>(p.iso);
+    const V vg = value_at<V>(p, uint32_t(i));
+    const bool below_g = vg < iso;
+    constexpr uint32_t corner[7] = { 1u, 4u, 3u, 2u, 5u, 6u, 7u };
+    bool found[7];
+    uint32_t pos[7];
+    bool inx, iny, inz;
+    if constexpr (std::is_same_v<Lookup, BitmapLookup>)
+    {
+        inx = r.x + 1 < g.dims[0];
+        iny = r.y + 1 < g.dims[1];
+        inz = r.z + 1 < g.dims[2];
+        // rows (dy, dz) = (0, 0), (1, 0), (0, 1), (1, 1); row (0, 0) holds the vertex itself
+        const RowProbe q00 = row_probe(lk, g, r.x, r.y, r.z, true, inx);
+        const RowProbe q10 = row_probe(lk, g, r.x, r.y + 1, r.z, iny, inx);
+        const RowProbe q01 = row_probe(lk, g, r.x, r.y, r.z + 1, inz, inx);
+        const RowProbe q11 = row_probe(lk, g, r.x, r.y + 1, r.z + 1, iny && inz, inx);
+        // corners 1 (x+1,y,z), 4 (x,y+1,z), 3 (x,y,z+1), 2 (x+1,y,z+1), 5 (x+1,y+1,z), 6 (x+1,y+1,z+1), 7 (x,y+1,z+1)
+        found[0] = q00.p1;
+        found[1] = q10.p0;
+        found[2] = q01.p0;
+        found[3] = q01.p1;
+        found[4] = q10.p1;
+        found[5] = q11.p1;
+        found[6] = q11.p0;
+        bool complete = inx && iny && inz; // MC:199
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            complete = complete && found[k];
+        pos[0] = uint32_t(i) + 1u; // the +x neighbour of a present vertex follows it
+        if (found[1] || complete)
+            row_ranks(lk, q10, r.x, pos[1], pos[4]);
+        if (found[2] || complete)
+            row_ranks(lk, q01, r.x, pos[2], pos[3]);
+        if (complete)
+            row_ranks(lk, q11, r.x, pos[6], pos[5]);
+        bool below[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            below[k] = (k < 3 ? found[k] : complete) && value_at<V>(p, pos[k]) < iso;
+        uint32_t sc = 0u;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (found[d] && below[d] != below_g)
+                sc |= 1u << d;
+        uint32_t ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
+        if (complete)
+        {
+            ci = below_g ? 1u : 0u;
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                ci |= below[k] ? (1u << corner[k]) : 0u;
+        }
+        info = ci | (complete ? kInfoComplete : 0u) | (sc << kInfoSignShift);
+        ntv = complete ? uint32_t(c_cases.count[ci]) : 0u;
+    }
+    else
+    {
+        typename Lookup::Probe pr[7];
+        cell_corners(lk, g, i, r, found, pr, inx, iny, inz);
+        bool complete = inx && iny && inz; // MC:199
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            complete = complete && found[k];
+        // values: the axis neighbours when present, the other corners of a complete cell
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (k < 3 ? found[k] : complete)
+                pos[k] = lk.rank(pr[k]);
+        bool below[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            below[k] = (k < 3 ? found[k] : complete) && value_at<V>(p, pos[k]) < iso;
+        uint32_t sc = 0u;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (found[d] && below[d] != below_g)
+                sc |= 1u << d;
+        uint32_t ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
+        if (complete)
+        {
+            ci = below_g ? 1u : 0u;
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                ci |= below[k] ? (1u << corner[k]) : 0u;
+        }
+        info = ci | (complete ? kInfoComplete : 0u) | (sc << kInfoSignShift);
+        ntv = complete ? uint32_t(c_cases.count[ci]) : 0u;
+    }
+}
+
 template <typename Lookup, typename V>
 __global__ __launch_bounds__(256) void
 mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* __restrict__ ntv)
@@ -205,56 +371,14 @@ mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* _
     if (i >= p.nverts)
         return;
     const VertexRef r = vertex_ref(p, i);
-    const McGrid& g = p.grids[r.scene];
-    const V iso = static_cast<V>(p.iso);
-    const V vg = value_at<V>(p, uint32_t(i));
-    const bool below_g = vg < iso;
-    // the cell's other seven corners, case-corner order 1, 4, 3 (the +x, +y, +z neighbours), then 2, 5, 6, 7
-    constexpr uint32_t corner[7] = { 1u, 4u, 3u, 2u, 5u, 6u, 7u };
-    const bool inx = r.x + 1 < g.dims[0], iny = r.y + 1 < g.dims[1], inz = r.z + 1 < g.dims[2];
-    typename Lookup::Probe pr[7];
-    bool in[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-    {
-        const uint32_t o = c_corner_off_host[corner[k]];
-        in[k] = (!(o & 1u) || inx) && (!((o >> 1) & 1u) || iny) && (!((o >> 2) & 1u) || inz);
-        if (in[k])
-            pr[k] = lk.probe(g, r.x + (o & 1u), r.y + ((o >> 1) & 1u), r.z + ((o >> 2) & 1u));
-    }
-    bool found[7];
-    bool complete = inx && iny && inz; // MC:199
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-    {
-        found[k] = in[k] && lk.present(pr[k]);
-        complete = complete && found[k];
-    }
-    // values: the axis neighbours when present, the other corners of a complete cell
-    uint32_t pos[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        if (k < 3 ? found[k] : complete)
-            pos[k] = lk.rank(pr[k]);
-    bool below[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        below[k] = (k < 3 ? found[k] : complete) && value_at<V>(p, pos[k]) < iso;
-    uint32_t sc = 0u;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-        if (found[d] && below[d] != below_g)
-            sc |= 1u << d;
-    uint32_t ci = 0u; // missing corner or boundary vertex: cubeindex 0 (MC:206-211)
-    if (complete)
-    {
-        ci = below_g ? 1u : 0u;
-#pragma unroll
-        for (int k = 0; k < 7; ++k)
-            ci |= below[k] ? (1u << corner[k]) : 0u;
-    }
-    info[i] = ci | (complete ? kInfoComplete : 0u) | (sc << kInfoSignShift);
-    ntv[i] = complete ? uint32_t(c_cases.count[ci]) : 0u;
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.scene);
+    uint32_t inf, n;
+    if (__all(r.scene == s0)) // one grid for the whole wave: scalar grid parameters
+        mc_classify_vertex<Lookup, V, true>(p, lk, wave_uniform(p.grids[s0]), i, r, inf, n);
+    else
+        mc_classify_vertex<Lookup, V, false>(p, lk, p.grids[r.scene], i, r, inf, n);
+    info[i] = inf;
+    ntv[i] = n;
 }
 
 // ---------------------------------------------------------------------------
