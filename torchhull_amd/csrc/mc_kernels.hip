@@ -464,16 +464,18 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
         const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
         const uint32_t p1[3] = { r.x, r.y, r.z };
         // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
+        // (bitmap policy: the +x endpoint of a present vertex i is vertex i + 1)
+        constexpr bool kBitmap = std::is_same_v<Lookup, BitmapLookup>;
         typename Lookup::Probe pr[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int d = kBitmap ? 1 : 0; d < 3; ++d)
             if (m & (1u << d))
                 pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
         float v2[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
             if (m & (1u << d))
-                v2[d] = float(value_at<V>(p, lk.rank(pr[d])));
+                v2[d] = float(value_at<V>(p, (kBitmap && d == 0) ? uint32_t(i) + 1u : lk.rank(pr[d])));
         uint32_t o = out - V0;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -588,17 +590,44 @@ mc_faces_unique_kernel(McPlan p,
         // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
         // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
         const uint32_t lows = c_cases.lows[ci];
-        typename Lookup::Probe pr[8];
-#pragma unroll
-        for (int o = 1; o < 8; ++o)
-            if (lows & (1u << o))
-                pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
         uint32_t pos[8];
         pos[0] = uint32_t(i);
+        if constexpr (std::is_same_v<Lookup, BitmapLookup>)
+        {
+            // corners 2r and 2r + 1 (x and x + 1) share row r = (dy, dz): one bitmap word and one
+            // prefix per needed row; the +x neighbour of vertex i is i + 1
+            pos[1] = uint32_t(i) + 1u;
+            uint64_t w[4];
+            uint32_t word[4], b = r.x & 31u;
 #pragma unroll
-        for (int o = 1; o < 8; ++o)
-            if (lows & (1u << o))
-                pos[o] = lk.rank(pr[o]);
+            for (int q = 1; q < 4; ++q)
+                if (lows & (3u << (2 * q)))
+                {
+                    w[q] = g.bitmap_offset + (uint64_t(r.z + (q >> 1)) * g.dims[1] + (r.y + (q & 1))) * g.row_words +
+                           (r.x >> 5);
+                    word[q] = lk.bitmap[w[q]];
+                }
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+                if (lows & (3u << (2 * q)))
+                {
+                    // every corner of a complete cell is present
+                    pos[2 * q] = lk.prefix[w[q]] + __popc(word[q] & ((1u << b) - 1u));
+                    pos[2 * q + 1] = b != 31u ? pos[2 * q] + 1u : lk.prefix[w[q] + 1];
+                }
+        }
+        else
+        {
+            typename Lookup::Probe pr[8];
+#pragma unroll
+            for (int o = 1; o < 8; ++o)
+                if (lows & (1u << o))
+                    pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
+#pragma unroll
+            for (int o = 1; o < 8; ++o)
+                if (lows & (1u << o))
+                    pos[o] = lk.rank(pr[o]);
+        }
         uint32_t vb[8], em[8];
 #pragma unroll
         for (int o = 0; o < 8; ++o)
