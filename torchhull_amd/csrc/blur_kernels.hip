@@ -297,8 +297,9 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     const int32_t x = x0 + lx;
     const bool dense = !sparse && k > 1;
     const bool use_flags = sparse && k > 1;
-    // pixel q of this thread: (x, y0 + ly0 + 4q); rows past H count as absent
-    int32_t nq = (H - (y0 + ly0) + 3) / 4;
+    // pixel q of this thread: (x, y0 + ly0 P + q), P consecutive rows; rows past H count as absent
+    const int32_t yt = y0 + ly0 * P;
+    int32_t nq = H - yt;
     nq = x < W ? min(max(nq, 0), P) : 0;
     uint32_t band = 0u;
     if (dense)
@@ -337,7 +338,7 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
         const int32_t bx_t = x / ts - tx_lo;
 #pragma unroll
         for (int q = 0; q < P; ++q)
-            if (q < nq && btl[((y0 + ly0 + 4 * q) / ts - ty_lo) * bw + bx_t] != 0)
+            if (q < nq && btl[((yt + q) / ts - ty_lo) * bw + bx_t] != 0)
                 band |= 1u << q;
     }
     if (!__syncthreads_or(band != 0u))
@@ -368,11 +369,11 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
 #pragma unroll
         for (int q = 0; q < P; ++q)
             if (q < nq)
-                v[q] = convert_in<To>(ld(I, int64_t(y0 + ly0 + 4 * q) * W + x)); // images.clone().to(dtype) (GB:444)
+                v[q] = convert_in<To>(ld(I, int64_t(yt + q) * W + x)); // images.clone().to(dtype) (GB:444)
 #pragma unroll
         for (int q = 0; q < P; ++q)
             if (q < nq)
-                st(O, int64_t(y0 + ly0 + 4 * q) * W + x, v[q]);
+                st(O, int64_t(yt + q) * W + x, v[q]);
         return;
     }
     // stage the block and its halo, converted to the output type (GB:221): thread (lx, ly0) stages
@@ -416,9 +417,47 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     __syncthreads();
     if (x >= W)
         return;
+    if constexpr (KT > 0)
+    {
+        // GB:217-230 per pixel: x offset i outer, y offset j inner.  The thread's P pixels are
+        // consecutive rows of one column, so for each i the column's P + KT - 1 tile values are read
+        // once and every pixel's accumulator takes its KT taps from them, in the same order.
+        To acc[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            acc[q] = OutOps<To>::zero();
+        if (band != 0u)
+        {
+#pragma unroll
+            for (int32_t i = 0; i < KT; ++i)
+            {
+                const To* col = tile + (lx + i) + (ly0 * P) * LW;
+                To c[P + KT - 1];
+#pragma unroll
+                for (int t = 0; t < P + KT - 1; ++t)
+                    c[t] = col[t * LW];
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+#pragma unroll
+                    for (int32_t jj = 0; jj < KT; ++jj)
+                        acc[q] = OutOps<To>::mac(acc[q], weights[i * KT + jj], c[q + jj]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+        {
+            if (q >= nq)
+                break;
+            const int32_t ly = ly0 * P + q;
+            const To result = (band & (1u << q)) ? (sparse ? OutOps<To>::div(acc[q], norm) : acc[q])
+                                                 : tile[(ly + h) * LW + lx + h];
+            st(O, int64_t(y0 + ly) * W + x, result);
+        }
+        return;
+    }
     for (int32_t q = 0; q < nq; ++q)
     {
-        const int32_t ly = ly0 + 4 * q;
+        const int32_t ly = ly0 * P + q;
         const int32_t y = y0 + ly;
         To result;
         if (!(band & (1u << q)))
@@ -427,25 +466,13 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
         {
             // GB:217-230: x offset i outer, y offset j inner
             To acc = OutOps<To>::zero();
-#pragma unroll
-            for (int32_t i = 0; i < (KT > 0 ? KT : 1); ++i)
+            for (int32_t i = 0; i < k; ++i)
             {
-                if (KT > 0)
-                {
-                    const To* col = tile + (lx + i) + ly * LW;
-#pragma unroll
-                    for (int32_t jj = 0; jj < KT; ++jj)
-                        acc = OutOps<To>::mac(acc, weights[i * KT + jj], col[jj * LW]);
-                }
+                const To* col = tile + (lx + i) + ly * LW;
+                const To* wcol = weights + i * k;
+                for (int32_t jj = 0; jj < k; ++jj)
+                    acc = OutOps<To>::mac(acc, wcol[jj], col[jj * LW]);
             }
-            if (KT == 0)
-                for (int32_t i = 0; i < k; ++i)
-                {
-                    const To* col = tile + (lx + i) + ly * LW;
-                    const To* wcol = weights + i * k;
-                    for (int32_t jj = 0; jj < k; ++jj)
-                        acc = OutOps<To>::mac(acc, wcol[jj], col[jj * LW]);
-                }
             result = sparse ? OutOps<To>::div(acc, norm) : acc;
         }
         st(O, int64_t(y) * W + x, result);
