@@ -297,6 +297,24 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     const int32_t x = x0 + lx;
     const bool dense = !sparse && k > 1;
     const bool use_flags = sparse && k > 1;
+    // float32 rows of a multiple of 4 pixels, 16-byte aligned: the block's own pixels are read with
+    // 16-byte loads before the band test (thread t: columns x0 + 4 (t % 16) .. + 3 of rows
+    // y0 + t / 16 + 16 r), so that the reads overlap the edge-flag reads; a block without band
+    // pixels stores them as they are
+    constexpr bool kF32 = std::is_same_v<Ti, float> && std::is_same_v<To, float>;
+    constexpr int R4 = RB / 16;
+    const int32_t cx4 = x0 + 4 * int32_t(threadIdx.x & 15), ry4 = y0 + int32_t(threadIdx.x >> 4);
+    float4_v pre[R4];
+    if constexpr (kF32)
+    {
+        if (vec4 && cx4 < W)
+        {
+#pragma unroll
+            for (int r = 0; r < R4; ++r)
+                if (ry4 + 16 * r < H)
+                    pre[r] = *(gptr<const float4_v>)(I + int64_t(ry4 + 16 * r) * W + cx4);
+        }
+    }
     // pixel q of this thread: (x, y0 + ly0 P + q), P consecutive rows; rows past H count as absent
     const int32_t yt = y0 + ly0 * P;
     int32_t nq = H - yt;
@@ -343,25 +361,16 @@ blur_tile_kernel(const BlurJob* __restrict__ jobs,
     }
     if (!__syncthreads_or(band != 0u))
     {
-        if constexpr (std::is_same_v<Ti, float> && std::is_same_v<To, float>)
+        if constexpr (kF32)
         {
             if (vec4)
             {
-                // float32 rows of a multiple of 4 pixels, 16-byte aligned: thread t copies 4 columns
-                // (x0 + 4 (t % 16) ...) of rows y0 + t / 16 + 16 r with 16-byte loads and stores
-                constexpr int R4 = RB / 16;
-                const int32_t cx = x0 + 4 * int32_t(threadIdx.x & 15), ry = y0 + int32_t(threadIdx.x >> 4);
-                if (cx >= W)
+                if (cx4 >= W)
                     return;
-                float4_v v[R4];
 #pragma unroll
                 for (int r = 0; r < R4; ++r)
-                    if (ry + 16 * r < H)
-                        v[r] = *(gptr<const float4_v>)(I + int64_t(ry + 16 * r) * W + cx);
-#pragma unroll
-                for (int r = 0; r < R4; ++r)
-                    if (ry + 16 * r < H)
-                        *(gptr<float4_v>)(O + int64_t(ry + 16 * r) * W + cx) = v[r];
+                    if (ry4 + 16 * r < H)
+                        *(gptr<float4_v>)(O + int64_t(ry4 + 16 * r) * W + cx4) = pre[r];
                 return;
             }
         }
