@@ -71,7 +71,6 @@ struct IntegralJob
     int32_t dtype;
     int32_t B, H, W;
     int32_t strips;       // ceil(H / kStripRows)
-    int32_t vec4;         // float32 masks, W % 4 == 0, 16-byte aligned: 4 columns per thread in the carry pass
     uint32_t col_blocks;  // carry-pass blocks per camera
     uint64_t col_begin;   // first block of this job in the carry pass
     uint64_t strip_begin; // first block of this job in the strip pass (one block per camera and strip)
@@ -108,36 +107,6 @@ carry_column(gptr<const T> src, gptr<float> carry, int32_t strips, int32_t W)
     }
 }
 
-__device__ __forceinline__ void
-carry_column4(gptr<const float> src, gptr<float> carry, int32_t strips, int32_t W)
-{
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    for (int32_t s = 0; s + 1 < strips; ++s)
-    {
-#pragma unroll
-        for (int h = 0; h < kStripRows; h += 8)
-        {
-            float4 v[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                v[k] = ld_float4(src + int64_t(s * kStripRows + h + k) * W);
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-            {
-                a0 = a0 + v[k].x;
-                a1 = a1 + v[k].y;
-                a2 = a2 + v[k].z;
-                a3 = a3 + v[k].w;
-            }
-        }
-        const auto c = carry + int64_t(s) * W;
-        c[0] = a0;
-        c[1] = a1;
-        c[2] = a2;
-        c[3] = a3;
-    }
-}
-
 __global__ __launch_bounds__(256) void
 integral_carry_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
 {
@@ -146,14 +115,12 @@ integral_carry_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     const uint64_t local = blk - j.col_begin;
     const int32_t b = int32_t(local / j.col_blocks);
     const int32_t t = int32_t(local - uint64_t(b) * j.col_blocks) * 256 + threadIdx.x;
-    const int32_t x = j.vec4 ? 4 * t : t;
+    const int32_t x = t; // one column per thread: many waves walk their columns at once
     if (x >= j.W)
         return;
     const int64_t in_off = int64_t(b) * j.H * j.W + x;
     const auto carry = as_global(j.carry + int64_t(b) * (j.strips - 1) * j.W + x);
-    if (j.vec4)
-        carry_column4(as_global(static_cast<const float*>(j.masks) + in_off), carry, j.strips, j.W);
-    else if (j.dtype == kF32)
+    if (j.dtype == kF32)
         carry_column(as_global(static_cast<const float*>(j.masks) + in_off), carry, j.strips, j.W);
     else
         carry_column(as_global(static_cast<const uint8_t*>(j.masks) + in_off), carry, j.strips, j.W);
@@ -200,17 +167,35 @@ integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripR
         {
             const int32_t cols = min(kChunk, W - c0);
             int32_t c = 0;
-            for (; c + 8 <= cols; c += 8)
+            // the next 8 values are read while the chain runs through the current 8
+            float v[8];
+            if (cols >= 8)
             {
-                float v[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    v[k] = tile[t][c + k];
+                    v[k] = tile[t][k];
+            }
+            for (; c + 8 <= cols; c += 8)
+            {
+                float nv[8];
+                const bool more = c + 16 <= cols;
+                if (more)
+                {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        nv[k] = tile[t][c + 8 + k];
+                }
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                 {
                     racc = racc + v[k];
                     tile[t][c + k] = racc;
+                }
+                if (more)
+                {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        v[k] = nv[k];
                 }
             }
             for (; c < cols; ++c)
@@ -277,7 +262,6 @@ make_job(const void* masks, int32_t dtype, int64_t B, int64_t H, int64_t W, floa
     j.H = int32_t(H);
     j.W = int32_t(W);
     j.strips = int32_t((H + kStripRows - 1) / kStripRows);
-    j.vec4 = dtype == kF32 && W % 4 == 0 && reinterpret_cast<uintptr_t>(masks) % 16 == 0;
     return j;
 }
 
@@ -302,7 +286,7 @@ integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* ca
     uint64_t cols = 0, strips = 0, cams = 0, coff = 0;
     for (auto& j : jobs)
     {
-        j.col_blocks = blocks_for(j.vec4 ? j.W / 4 : j.W, 256);
+        j.col_blocks = blocks_for(j.W, 256);
         j.col_begin = cols;
         j.strip_begin = strips;
         j.cam_begin = cams;
