@@ -53,6 +53,15 @@ int thull_ctx_memory(thull_ctx* ctx, int64_t* peak_bytes, int64_t* reserved_byte
 /* Release pooled memory that holds no result. */
 int thull_ctx_trim(thull_ctx* ctx);
 
+/* Carving launch shape by octree level size, with no counterpart in the reference
+ * (its carving launches one thread per node): a level of at most max_parents[0]
+ * parents splits each parent's cameras over 64 lanes, up to max_parents[1] over 16,
+ * up to max_parents[2] over 4, larger levels use one lane per parent.  Values are
+ * non-decreasing; the results do not depend on them.  torchhull_amd.tuning measures
+ * and stores the fastest values for a device. */
+int thull_ctx_set_carve_tuning(thull_ctx* ctx, const uint32_t* max_parents);
+int thull_ctx_get_carve_tuning(thull_ctx* ctx, uint32_t* max_parents);
+
 /* One visual-hull problem (the arguments of torchhull.visual_hull,
  * src/torchhull/_C/python/bindings.cpp:12-21). */
 typedef struct thull_scene

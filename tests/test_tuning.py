@@ -1,0 +1,79 @@
+"""Carving launch-shape tuning: the threshold rule, the threshold choice and the stored file (no GPU needed),
+and that every launch shape gives the same mesh (GPU)."""
+
+import json
+
+import pytest
+import torch
+
+from torchhull_amd import tuning
+
+
+def test_split_rule_matches_thresholds():
+    t = (100, 1000, 10000)
+    assert [tuning.split_for(n, t) for n in (1, 100, 101, 1000, 1001, 10000, 10001)] == [64, 64, 16, 16, 4, 4, 1]
+
+
+def test_choice_minimises_summed_level_time():
+    # three levels; the fastest G per level is 64, 4 and 1 (a monotone choice exists)
+    sizes = [50, 5000, 500000]
+    ms = {64: [1.0, 9.0, 90.0], 16: [2.0, 5.0, 40.0], 4: [3.0, 2.0, 20.0], 1: [9.0, 8.0, 10.0]}
+    samples = {g: list(zip(sizes, v)) for g, v in ms.items()}
+    chosen, total = tuning.choose_thresholds(samples)
+    assert total == pytest.approx(1.0 + 2.0 + 10.0)
+    assert [tuning.split_for(n, chosen) for n in sizes] == [64, 4, 1]
+
+
+def test_choice_is_monotone_when_times_are_not():
+    # the best per-level G (1, 64) is not monotone in the level size; the rule can only pick monotone shapes
+    sizes = [10, 20]
+    ms = {64: [5.0, 1.0], 16: [5.0, 5.0], 4: [5.0, 5.0], 1: [1.0, 5.0]}
+    chosen, total = tuning.choose_thresholds({g: list(zip(sizes, v)) for g, v in ms.items()})
+    assert total == pytest.approx(6.0)
+    g = [tuning.split_for(n, chosen) for n in sizes]
+    assert g[0] >= g[1]
+
+
+def test_load_accepts_only_valid_files(tmp_path):
+    p = tmp_path / "t.json"
+    assert tuning.load(p) is None
+    p.write_text(json.dumps({"max_parents": [1, 2, 3]}))
+    assert tuning.load(p) == [1, 2, 3]
+    for bad in ([3, 2, 1], [1, 2], [-1, 2, 3], ["a", 2, 3]):
+        p.write_text(json.dumps({"max_parents": bad}))
+        assert tuning.load(p) is None
+    p.write_text("not json")
+    assert tuning.load(p) is None
+
+
+def test_stored_tuning_file_is_valid():
+    if tuning.DEFAULT_PATH.exists():
+        assert tuning.load() is not None
+
+
+@pytest.mark.gpu
+def test_every_carving_launch_shape_gives_the_same_mesh():
+    import torchhull_amd as th
+    from torchhull_amd import _lib, synthetic
+
+    dev = torch.device("cuda:0")
+    ctx = _lib.context(0)
+    saved = tuning.get_carve_tuning(ctx)
+    scenes = synthetic.mixed_scenes(3, seed=77, device=dev, max_level=7)
+    blurred = [th.gaussian_blur(sc.masks, sc.blur[0], sc.blur[1], sparse=True, dtype=torch.float32) for sc in scenes]
+    meshes = []
+    try:
+        for force in tuning._FORCE.values():
+            tuning.set_carve_tuning(ctx, force)
+            out = []
+            for sc, m in zip(scenes, blurred):
+                v, f = th.visual_hull(m, sc.transforms, sc.level, sc.cube_corner_bfl, sc.cube_length,
+                                      sc.masks_partial, sc.transforms_convention, unique_verts=True)
+                out.append((v.cpu(), f.cpu()))
+            meshes.append(out)
+    finally:
+        tuning.set_carve_tuning(ctx, saved)
+    for other in meshes[1:]:
+        for (v0, f0), (v1, f1) in zip(meshes[0], other):
+            assert torch.equal(f0, f1)
+            assert torch.equal(v0.view(torch.int32), v1.view(torch.int32))

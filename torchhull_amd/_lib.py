@@ -63,6 +63,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.thull_ctx_destroy.argtypes = [c_void_p]
     L.thull_ctx_memory.argtypes = [c_void_p, P_i64, P_i64]
     L.thull_ctx_trim.argtypes = [c_void_p]
+    L.thull_ctx_set_carve_tuning.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+    L.thull_ctx_get_carve_tuning.argtypes = [c_void_p, ctypes.POINTER(ctypes.c_uint32)]
     L.thull_visual_hull_run.argtypes = [c_void_p, ctypes.POINTER(Scene), c_i32, c_i32, c_void_p,
                                         ctypes.POINTER(VhCounts)]
     L.thull_visual_hull_fetch_mesh.argtypes = [c_void_p, c_i32, c_void_p, c_void_p, c_void_p]
@@ -125,4 +127,9 @@ def context(device_index: int) -> c_void_p:
         ctx = c_void_p()
         check(lib().thull_ctx_create(int(device_index), ctypes.byref(ctx)))
         ctxs[device_index] = ctx
+        from . import tuning  # noqa: PLC0415 - tuning imports this module
+
+        stored = tuning.load()
+        if stored is not None:  # carving launch shape measured by torchhull_amd.tuning
+            tuning.set_carve_tuning(ctx, stored)
     return ctx

@@ -298,6 +298,7 @@ struct Context
     HostStage* stage = nullptr;
     UploadRing upload;
     HostNotify notify;
+    CarveTuning tuning;
     hipStream_t last_stream = nullptr;
     bool have_last = false;
 
@@ -606,7 +607,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         uint32_t* sums = pool.alloc<uint32_t>(scan_blocks(nb) + 1);
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i), st));
-        const int32_t split = launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests + i, st);
+        const int32_t split = launch_carve_classify(list, count, i, scenes, mask, bcount, dev_tests + i, ctx.tuning, st);
         if (ctx.profile)
             THULL_HIP(hipEventRecord(ctx.event(2 * i + 1), st));
         // the scan sends the next level's size (and the camera tests) to the host; the children go out
@@ -865,6 +866,28 @@ thull_ctx_memory(thull_ctx* ctx, int64_t* peak, int64_t* reserved)
             *peak = int64_t(ctx->c.pool.peak());
         if (reserved)
             *reserved = int64_t(ctx->c.pool.reserved());
+    });
+}
+
+int
+thull_ctx_set_carve_tuning(thull_ctx* ctx, const uint32_t* max_parents)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr && max_parents != nullptr, "ctx and max_parents must not be null");
+        THULL_CHECK(max_parents[0] <= max_parents[1] && max_parents[1] <= max_parents[2],
+                    "max_parents must be non-decreasing");
+        for (int k = 0; k < 3; ++k)
+            ctx->c.tuning.max_parents[k] = max_parents[k];
+    });
+}
+
+int
+thull_ctx_get_carve_tuning(thull_ctx* ctx, uint32_t* max_parents)
+{
+    return guarded([&] {
+        THULL_CHECK(ctx != nullptr && max_parents != nullptr, "ctx and max_parents must not be null");
+        for (int k = 0; k < 3; ++k)
+            max_parents[k] = ctx->c.tuning.max_parents[k];
     });
 }
 

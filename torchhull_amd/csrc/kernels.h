@@ -37,9 +37,16 @@ constexpr int kCarveThreads = 256;
 // parents.  `tests` accumulates the number of camera tests (early exit).
 // Returns the number of lanes that share one parent's cameras (1: one thread
 // per parent, carve_classify_kernel; 4/16/64: carve_classify_split_kernel<G>).
+// Lanes per parent by level size: G = 64 for at most max_parents[0] parents,
+// 16 up to max_parents[1], 4 up to max_parents[2], else 1.
+struct CarveTuning
+{
+    uint32_t max_parents[3] = { 16384u, 131072u, 1048576u };
+};
+int32_t carve_split_factor(uint32_t n, const CarveTuning& tune);
 int32_t launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level_i, const SceneDev* scenes,
-                              uint8_t* child_mask, uint32_t* block_counts, unsigned long long* tests, hipStream_t st);
-int32_t carve_split_factor(uint32_t n);
+                              uint8_t* child_mask, uint32_t* block_counts, unsigned long long* tests,
+                              const CarveTuning& tune, hipStream_t st);
 
 // Writes the occupied children of every parent, parent-major and child-minor,
 // at `block_offsets[block]` + the in-block prefix.  With `cells`, the

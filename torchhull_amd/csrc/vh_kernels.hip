@@ -1044,11 +1044,12 @@ launch_carve_classify(const uint64_t* parents,
                       uint8_t* child_mask,
                       uint32_t* block_counts,
                       unsigned long long* tests,
+                      const CarveTuning& tune,
                       hipStream_t st)
 {
     if (n == 0)
         return 0;
-    const int32_t split = carve_split_factor(n);
+    const int32_t split = carve_split_factor(n, tune);
     const uint64_t nb = blocks_for(n, kCarveThreads);
     if (split == 64)
     {
@@ -1081,14 +1082,14 @@ launch_carve_classify(const uint64_t* parents,
 }
 
 int32_t
-carve_split_factor(uint32_t n)
+carve_split_factor(uint32_t n, const CarveTuning& tune)
 {
-    // small levels split each parent's cameras over G lanes to fill the chip (about 10^6 lanes)
-    if (n <= 16384)
+    // small levels split each parent's cameras over G lanes to fill the chip
+    if (n <= tune.max_parents[0])
         return 64;
-    if (n <= 131072)
+    if (n <= tune.max_parents[1])
         return 16;
-    if (n <= 1048576)
+    if (n <= tune.max_parents[2])
         return 4;
     return 1;
 }

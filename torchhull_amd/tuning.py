@@ -1,0 +1,158 @@
+"""Autotuner of the carving launch shape (lanes per parent by octree level size).
+
+Carving classifies each level in one launch.  Small levels split a parent's
+cameras over G lanes (64, 16 or 4) to fill the GPU; large ones use one lane per
+parent.  Which G is fastest for a level of n parents depends on the device and
+on the scenes (cameras per scene, early exits), so it is measured:
+`tune_carving` runs a workload once per G with that G forced on every level,
+reads every level's classification time (HIP events, `thull_profile_carve`),
+and picks the three level-size thresholds that minimise the summed time of
+all measured levels.  The choice never changes a result, only its speed.
+
+`python -m torchhull_amd.tuning --out torchhull_amd/carve_tuning.json` tunes on
+the mixed 64-scene workload and the 32-camera 2048^2 level-10 scene and writes
+the thresholds; `_lib.context` applies a stored file to every new context.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import itertools
+import json
+import pathlib
+import statistics
+import time
+from typing import Callable, Sequence
+
+import torch
+
+from . import _lib
+
+DEFAULT_PATH = pathlib.Path(__file__).resolve().parent / "carve_tuning.json"
+_MAX = 0xFFFFFFFF
+# thresholds that force one G on every level
+_FORCE = {64: (_MAX, _MAX, _MAX), 16: (0, _MAX, _MAX), 4: (0, 0, _MAX), 1: (0, 0, 0)}
+
+
+def set_carve_tuning(ctx: ctypes.c_void_p, max_parents: Sequence[int]) -> None:
+    arr = (ctypes.c_uint32 * 3)(*[int(v) for v in max_parents])
+    _lib.check(_lib.lib().thull_ctx_set_carve_tuning(ctx, arr))
+
+
+def get_carve_tuning(ctx: ctypes.c_void_p) -> list[int]:
+    arr = (ctypes.c_uint32 * 3)()
+    _lib.check(_lib.lib().thull_ctx_get_carve_tuning(ctx, arr))
+    return list(arr)
+
+
+def split_for(n: int, max_parents: Sequence[int]) -> int:
+    """Lanes per parent for a level of n parents (the library's rule)."""
+    if n <= max_parents[0]:
+        return 64
+    if n <= max_parents[1]:
+        return 16
+    if n <= max_parents[2]:
+        return 4
+    return 1
+
+
+def measure_levels(run: Callable[[], object], ctx: ctypes.c_void_p, reps: int = 3) -> list[tuple[int, float]]:
+    """(parents, median classification ms) of every carving step of run()."""
+    L = _lib.lib()
+    n = 16
+    runs = []
+    for _ in range(reps):
+        _lib.check(L.thull_profile_enable(ctx, 1))
+        try:
+            run()
+            torch.cuda.synchronize()
+        finally:
+            _lib.check(L.thull_profile_enable(ctx, 0))
+        ms = (ctypes.c_float * n)()
+        par, tests, nxt = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+        split = (ctypes.c_int32 * n)()
+        _lib.check(L.thull_profile_carve(ctx, n, ms, par, tests, nxt, split))
+        runs.append([(int(par[i]), float(ms[i])) for i in range(n) if par[i] > 0])
+    return [(runs[0][i][0], statistics.median(r[i][1] for r in runs)) for i in range(len(runs[0]))]
+
+
+def choose_thresholds(samples: dict[int, list[tuple[int, float]]]) -> tuple[list[int], float]:
+    """Thresholds minimising the summed level time; samples[G] = [(parents, ms), ...] over the same levels."""
+    levels = samples[1]
+    cands = sorted({0, _MAX} | {n for n, _ in levels})
+    best, best_t = None, float("inf")
+    for t0, t1, t2 in itertools.combinations_with_replacement(cands, 3):
+        t = 0.0
+        for i, (n, _) in enumerate(levels):
+            t += samples[split_for(n, (t0, t1, t2))][i][1]
+        if t < best_t:
+            best, best_t = [t0, t1, t2], t
+    return best, best_t
+
+
+def tune_carving(workloads: Sequence[Callable[[], object]], device_index: int = 0, reps: int = 3) -> dict:
+    """Measures every G on every workload and returns the chosen thresholds with the measurements."""
+    ctx = _lib.context(device_index)
+    saved = get_carve_tuning(ctx)
+    samples: dict[int, list[tuple[int, float]]] = {g: [] for g in _FORCE}
+    try:
+        for run in workloads:
+            run()  # warm-up (pool growth, weights)
+            for g, force in _FORCE.items():
+                set_carve_tuning(ctx, force)
+                samples[g].extend(measure_levels(run, ctx, reps))
+    finally:
+        set_carve_tuning(ctx, saved)
+    chosen, total = choose_thresholds(samples)
+    current = sum(samples[split_for(n, saved)][i][1] for i, (n, _) in enumerate(samples[1]))
+    return {"max_parents": chosen, "predicted_ms": round(total, 4), "default_ms": round(current, 4),
+            "levels": [{"parents": n, **{f"ms_g{g}": round(samples[g][i][1], 4) for g in _FORCE}}
+                       for i, (n, _) in enumerate(samples[1])]}
+
+
+def load(path: pathlib.Path = DEFAULT_PATH) -> list[int] | None:
+    """Stored thresholds, or None when there is no usable file."""
+    try:
+        with open(path) as f:
+            v = json.load(f)["max_parents"]
+        if len(v) == 3 and all(isinstance(x, int) and 0 <= x <= _MAX for x in v) and v[0] <= v[1] <= v[2]:
+            return v
+    except (OSError, ValueError, KeyError, TypeError):
+        pass
+    return None
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", type=pathlib.Path, default=DEFAULT_PATH)
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args()
+    from . import api, synthetic
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    scenes = synthetic.mixed_scenes(args.scenes, device=dev)
+    blurred = api.gaussian_blur_batch([sc.masks for sc in scenes], scenes[0].blur[0], scenes[0].blur[1], sparse=True,
+                                      dtype=torch.float32)
+    batch = [dict(masks=m, transforms=sc.transforms, level=sc.level, cube_corner_bfl=sc.cube_corner_bfl,
+                  cube_length=sc.cube_length, masks_partial=sc.masks_partial,
+                  transforms_convention=sc.transforms_convention) for sc, m in zip(scenes, blurred)]
+    sc2 = synthetic.config_scene(2, 0, device=dev)
+    m2 = api.gaussian_blur(sc2.masks, sc2.blur[0], sc2.blur[1], sparse=True, dtype=torch.float32)
+    one = dict(masks=m2, transforms=sc2.transforms, level=sc2.level, cube_corner_bfl=sc2.cube_corner_bfl,
+               cube_length=sc2.cube_length, masks_partial=sc2.masks_partial,
+               transforms_convention=sc2.transforms_convention)
+    t0 = time.perf_counter()
+    res = tune_carving([lambda: api.visual_hull_batch(batch), lambda: api.visual_hull(**one)], reps=args.reps)
+    res["device"] = torch.cuda.get_device_name(dev)
+    res["workloads"] = [f"mixed{args.scenes} batched", "config 2 scene 0 (32 x 2048^2, level 10)"]
+    res["seconds"] = round(time.perf_counter() - t0, 1)
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps({k: res[k] for k in ("max_parents", "predicted_ms", "default_ms", "device", "seconds")}))
+
+
+if __name__ == "__main__":
+    main()
