@@ -54,3 +54,45 @@ def test_budget_smaller_than_one_request_runs_requests_alone():
     assert all(r.batch_size == 1 for r in res)
     for r, a in zip(res, alone):
         assert _same((r.verts, r.faces), a)
+
+
+@pytest.mark.gpu
+def test_contexts_of_finished_threads_are_released():
+    # every host thread gets its own library context with pooled device memory; a thread that ends
+    # releases it, so calls from many short-lived threads do not accumulate device memory
+    import gc
+    import threading
+
+    import torchhull_amd as th
+    from torchhull_amd import synthetic
+
+    dev = torch.device("cuda:0")
+    sc = synthetic.mixed_scenes(1, seed=91, device=dev, max_level=7)[0]
+    args = dict(masks=sc.masks, transforms=sc.transforms, level=sc.level, cube_corner_bfl=sc.cube_corner_bfl,
+                cube_length=sc.cube_length, masks_partial=sc.masks_partial,
+                transforms_convention=sc.transforms_convention)
+    ref = th.visual_hull(**args)
+
+    def work(out):
+        out.append(th.visual_hull(**args))
+        torch.cuda.synchronize()
+
+    def run_threads(n):
+        results = []
+        for _ in range(n):
+            t = threading.Thread(target=work, args=(results,))
+            t.start()
+            t.join()
+        gc.collect()
+        torch.cuda.synchronize()
+        return results
+
+    run_threads(2)
+    free0, _ = torch.cuda.mem_get_info(dev)
+    results = run_threads(12)
+    free1, _ = torch.cuda.mem_get_info(dev)
+    assert len(results) == 12
+    for v, f in results:
+        assert torch.equal(f, ref[1]) and torch.equal(v, ref[0])
+    # 12 leaked pools would hold at least 12 x 64 MiB
+    assert free0 - free1 < (256 << 20), (free0, free1)

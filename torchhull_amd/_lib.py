@@ -117,11 +117,26 @@ def check(status: int) -> None:
 _tls = threading.local()
 
 
+class _ThreadContexts(dict):
+    """device index -> context of one host thread; the contexts (and their pooled device memory) are
+    destroyed when the thread ends and its thread-local data goes away."""
+
+    def __del__(self) -> None:
+        L = _lib
+        if L is None:
+            return
+        for ctx in self.values():
+            try:
+                L.thull_ctx_destroy(ctx)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+
+
 def context(device_index: int) -> c_void_p:
     """Per-thread, per-device library context (pooled device memory)."""
     ctxs = getattr(_tls, "ctxs", None)
     if ctxs is None:
-        ctxs = _tls.ctxs = {}
+        ctxs = _tls.ctxs = _ThreadContexts()
     ctx = ctxs.get(device_index)
     if ctx is None:
         ctx = c_void_p()
