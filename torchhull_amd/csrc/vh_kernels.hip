@@ -1606,15 +1606,37 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
     f.i10 = y1in && x0in;
     f.i01 = y0in && x1in;
     f.i11 = y1in && x1in;
-    // in-plane offsets (H * W < 2^31), exact whenever the sample is read; a sample outside the image
-    // reads element 0 and is replaced by padding
-    const uint32_t o00 = iy * W + ix;
+    // in-plane offsets (H * W < 2^31); a sample outside the image reads an element inside and is
+    // replaced by padding
     typedef __attribute__((address_space(1))) const T global_t;
     global_t* M = (global_t*)(static_cast<const T*>(s.masks) + int64_t(b) * s.H * s.W);
-    f.v00 = M[f.i00 ? o00 : 0u];
-    f.v10 = M[f.i10 ? o00 + W : 0u];
-    f.v01 = M[f.i01 ? o00 + 1u : 0u];
-    f.v11 = M[f.i11 ? o00 + W + 1u : 0u];
+    if (W >= 2u)
+    {
+        // columns ix and ix + 1 of a row are read as one pair (one load of 2 elements, element
+        // alignment) starting at column xa = clamp(ix, 0, W - 2): for ix in [0, W - 2] the pair is
+        // (ix, ix + 1); ix = W - 1 is the pair's second element, ix = -1 makes ix + 1 = 0 its first.
+        // Rows outside the image read row 0.
+        typedef T pair_t __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+        typedef __attribute__((address_space(1))) const pair_t global_pair_t;
+        const uint32_t xa = uint32_t(min(max(int32_t(ix), 0), int32_t(W) - 2));
+        const uint32_t r0 = (y0in ? iy : 0u) * W + xa;
+        const uint32_t r1 = (y1in ? iy + 1u : 0u) * W + xa;
+        const pair_t p0 = *(global_pair_t*)(M + r0);
+        const pair_t p1 = *(global_pair_t*)(M + r1);
+        const bool first0 = ix == xa, first1 = ix + 1u == xa;
+        f.v00 = first0 ? p0.x : p0.y;
+        f.v10 = first0 ? p1.x : p1.y;
+        f.v01 = first1 ? p0.x : p0.y;
+        f.v11 = first1 ? p1.x : p1.y;
+    }
+    else
+    {
+        const uint32_t o00 = iy * W + ix;
+        f.v00 = M[f.i00 ? o00 : 0u];
+        f.v10 = M[f.i10 ? o00 + W : 0u];
+        f.v01 = M[f.i01 ? o00 + 1u : 0u];
+        f.v11 = M[f.i11 ? o00 + W + 1u : 0u];
+    }
     f.ty = py - fy;
     f.tx = px - fx;
 }
