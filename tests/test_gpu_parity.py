@@ -290,6 +290,22 @@ def test_sparse_blur_small_shapes_match_oracle(shape, k):
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
 
 
+def test_sparse_blur_mostly_flat_images_match_oracle():
+    # float32 images of width % 4 == 0 whose 64 x 64 blocks are mostly without band pixels (the edge
+    # pass copies them) next to an all-zero and an all-one image (no band at all)
+    th = _th()
+    rng = np.random.default_rng(11)
+    m = np.zeros((4, 200, 328, 1), np.float32)
+    yy, xx = np.mgrid[:200, :328]
+    m[0, ..., 0] = ((yy - 90) ** 2 + (xx - 150) ** 2 < 30 ** 2)
+    m[1, ..., 0] = (rng.uniform(size=(200, 328)) < 0.002)
+    m[3] = 1.0
+    for k in (3, 7):
+        got = th.gaussian_blur(torch.from_numpy(m).to(DEV), k, 1.5, sparse=True, dtype=torch.float32).cpu().numpy()
+        ref = oracle.gaussian_blur_sparse(m.astype(np.float64), k, 1.5, np.float32)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
 @pytest.mark.parametrize("k", [3, 9, 15])
 @pytest.mark.parametrize("sigma", [1.0, 100.0])
 def test_blur_dense_and_sparse_consistent(k, sigma):

@@ -1549,16 +1549,21 @@ int64_round_trip(float r)
 // combined by field_factor, so that the reads of several cameras are in
 // flight together.  `skip`: a partial-mask camera that does not see the point
 // well (VH:513-518) leaves the product unchanged.
+// PAIR (images at least 2 columns wide): each of the two rows is read as one pair of
+// adjacent columns.  The loaded values are only combined in field_factor, after the reads
+// of all cameras in flight have been issued.
 template <typename T>
 struct FieldSample
 {
-    T v00, v10, v01, v11;
+    typedef T pair_t __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+    pair_t p0, p1; // two adjacent columns of rows y0 and y1
     float ty, tx;
     bool i00, i10, i01, i11;
+    bool first0, first1; // column x0 (x0 + 1) is the pair's first element
     bool skip;
 };
 
-template <typename T>
+template <typename T, bool PAIR>
 __device__ __forceinline__ void
 field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSample<T>& f)
 {
@@ -1610,32 +1615,28 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
     // replaced by padding
     typedef __attribute__((address_space(1))) const T global_t;
     global_t* M = (global_t*)(static_cast<const T*>(s.masks) + int64_t(b) * s.H * s.W);
-    if (W >= 2u)
+    if constexpr (PAIR)
     {
         // columns ix and ix + 1 of a row are read as one pair (one load of 2 elements, element
         // alignment) starting at column xa = clamp(ix, 0, W - 2): for ix in [0, W - 2] the pair is
         // (ix, ix + 1); ix = W - 1 is the pair's second element, ix = -1 makes ix + 1 = 0 its first.
         // Rows outside the image read row 0.
-        typedef T pair_t __attribute__((ext_vector_type(2), aligned(sizeof(T))));
-        typedef __attribute__((address_space(1))) const pair_t global_pair_t;
+        typedef __attribute__((address_space(1))) const typename FieldSample<T>::pair_t global_pair_t;
         const uint32_t xa = uint32_t(min(max(int32_t(ix), 0), int32_t(W) - 2));
-        const uint32_t r0 = (y0in ? iy : 0u) * W + xa;
-        const uint32_t r1 = (y1in ? iy + 1u : 0u) * W + xa;
-        const pair_t p0 = *(global_pair_t*)(M + r0);
-        const pair_t p1 = *(global_pair_t*)(M + r1);
-        const bool first0 = ix == xa, first1 = ix + 1u == xa;
-        f.v00 = first0 ? p0.x : p0.y;
-        f.v10 = first0 ? p1.x : p1.y;
-        f.v01 = first1 ? p0.x : p0.y;
-        f.v11 = first1 ? p1.x : p1.y;
+        f.p0 = *(global_pair_t*)(M + ((y0in ? iy : 0u) * W + xa));
+        f.p1 = *(global_pair_t*)(M + ((y1in ? iy + 1u : 0u) * W + xa));
+        f.first0 = ix == xa;
+        f.first1 = ix + 1u == xa;
     }
     else
     {
         const uint32_t o00 = iy * W + ix;
-        f.v00 = M[f.i00 ? o00 : 0u];
-        f.v10 = M[f.i10 ? o00 + W : 0u];
-        f.v01 = M[f.i01 ? o00 + 1u : 0u];
-        f.v11 = M[f.i11 ? o00 + W + 1u : 0u];
+        f.p0.x = M[f.i00 ? o00 : 0u];
+        f.p0.y = M[f.i01 ? o00 + 1u : 0u];
+        f.p1.x = M[f.i10 ? o00 + W : 0u];
+        f.p1.y = M[f.i11 ? o00 + W + 1u : 0u];
+        f.first0 = true;
+        f.first1 = false;
     }
     f.ty = py - fy;
     f.tx = px - fx;
@@ -1645,10 +1646,10 @@ template <typename T>
 __device__ __forceinline__ float
 field_factor(const FieldSample<T>& f, float pad)
 {
-    const float v00 = f.i00 ? float(f.v00) : pad;
-    const float v10 = f.i10 ? float(f.v10) : pad;
-    const float v01 = f.i01 ? float(f.v01) : pad;
-    const float v11 = f.i11 ? float(f.v11) : pad;
+    const float v00 = f.i00 ? float(f.first0 ? f.p0.x : f.p0.y) : pad;
+    const float v10 = f.i10 ? float(f.first0 ? f.p1.x : f.p1.y) : pad;
+    const float v01 = f.i01 ? float(f.first1 ? f.p0.x : f.p0.y) : pad;
+    const float v11 = f.i11 ? float(f.first1 ? f.p1.x : f.p1.y) : pad;
     const float l0 = lerp_ref(v00, v10, f.ty);
     const float l1 = lerp_ref(v01, v11, f.ty);
     return lerp_ref(l0, l1, f.tx);
@@ -1657,14 +1658,10 @@ field_factor(const FieldSample<T>& f, float pad)
 // Product over the cameras, in camera order, of the bilinear mask samples at
 // a grid vertex (VH:470-530); masks of element type T (float32 or uint8).
 // UNIFORM as for classify_scene: keeps the wave-uniform and per-lane copies apart.
-template <typename T, bool UNIFORM>
+template <typename T, bool PAIR>
 __device__ __forceinline__ float
-field_value(const SceneDev& s, uint64_t key)
+field_product(const SceneDev& s, float wx, float wy, float wz)
 {
-    const float inv = __builtin_ldexpf(1.0f, -s.level);
-    const float wx = grid_to_world(key_x(key), inv, s.length, s.corner[0]);
-    const float wy = grid_to_world(key_y(key), inv, s.length, s.corner[1]);
-    const float wz = grid_to_world(key_z(key), inv, s.length, s.corner[2]);
     const float pad = s.partial != 0 ? 1.0f : 0.0f; // ones padding (partial) or zeros padding (full), IU:162-214
     constexpr int U = 4; // cameras whose reads are in flight together
     float value = 1.0f;
@@ -1674,7 +1671,7 @@ field_value(const SceneDev& s, uint64_t key)
         FieldSample<T> f[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            field_sample<T>(s, b + u, wx, wy, wz, f[u]);
+            field_sample<T, PAIR>(s, b + u, wx, wy, wz, f[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (!f[u].skip)
@@ -1683,11 +1680,22 @@ field_value(const SceneDev& s, uint64_t key)
     for (; b < s.B; ++b)
     {
         FieldSample<T> f;
-        field_sample<T>(s, b, wx, wy, wz, f);
+        field_sample<T, PAIR>(s, b, wx, wy, wz, f);
         if (!f.skip)
             value *= field_factor<T>(f, pad);
     }
     return value;
+}
+
+template <typename T, bool UNIFORM>
+__device__ __forceinline__ float
+field_value(const SceneDev& s, uint64_t key)
+{
+    const float inv = __builtin_ldexpf(1.0f, -s.level);
+    const float wx = grid_to_world(key_x(key), inv, s.length, s.corner[0]);
+    const float wy = grid_to_world(key_y(key), inv, s.length, s.corner[1]);
+    const float wz = grid_to_world(key_z(key), inv, s.length, s.corner[2]);
+    return s.W >= 2 ? field_product<T, true>(s, wx, wy, wz) : field_product<T, false>(s, wx, wy, wz);
 }
 
 __global__ __launch_bounds__(256) void
