@@ -40,7 +40,9 @@ def _lib():
 
 @pytest.mark.parametrize("kind", ["binary", "fractional"])
 @pytest.mark.parametrize("shape", [(3, 17, 33), (2, 256, 300), (1, 1024, 64), (2, 130, 100), (1, 1, 1), (1, 1, 4),
-                                   (1, 65, 4), (2, 64, 36), (1, 3, 2052)])
+                                   (1, 65, 4), (2, 64, 36), (1, 3, 2052),
+                                   # batches of at least 1024 images take the single-pass form
+                                   (1030, 70, 300), (1024, 33, 7)])
 def test_integral_image_matches_oracle_and_torch(kind, shape):
     rng = np.random.default_rng(1)
     if kind == "binary":
@@ -173,6 +175,22 @@ def test_blur_batch_equals_single():
     for im, o in zip(ims, outs):
         ref = th.gaussian_blur(im, 9, 2.0, sparse=True, dtype=torch.float32)
         assert torch.equal(o.view(torch.int32), ref.view(torch.int32))
+
+
+def test_visual_hull_batch_of_many_cameras_equals_single():
+    # 1000+ cameras in one batch: the integral images take the single-pass form, single scenes the two-pass one
+    th = _th()
+    sc = _scene("c0_l5")
+    m = _blurred(sc)
+    reps = -(-1024 // m.shape[0]) + 1
+    one = dict(masks=m, transforms=sc.transforms.to(DEV), level=sc.level, cube_corner_bfl=sc.cube_corner_bfl,
+               cube_length=sc.cube_length, masks_partial=sc.masks_partial,
+               transforms_convention=sc.transforms_convention)
+    v1, f1 = th.visual_hull(**one)
+    batch = th.visual_hull_batch([one] * reps)
+    assert reps * m.shape[0] >= 1024
+    for vb, fb in batch:
+        assert torch.equal(vb, v1) and torch.equal(fb, f1)
 
 
 @pytest.mark.parametrize("unique", [True, False])

@@ -6,6 +6,7 @@
 // loop and compaction VH:627-751, corner dedup VH:770-825, field VH:419-520
 // and VH:827-888.
 
+#include <algorithm>
 #include <cfloat>
 #include <vector>
 
@@ -54,6 +55,11 @@ as_float<__half>(__half v)
 //    kChunk-column chunk into LDS, runs the strip's row chains across the
 //    chunk (one lane per row), and writes the chunk out coalesced.  The next
 //    chunk's mask reads are in flight during the row chains.
+// Batches of many images take a single pass instead: one workgroup walks the
+// strips of one image from the top and stores each strip's column-chain
+// values itself, so the masks are read once (the carry pass reads them a
+// second time); the two-pass form keeps a small batch's strips spread over
+// the whole GPU.
 // ===========================================================================
 
 constexpr int kStripRows = 32;
@@ -126,7 +132,9 @@ integral_carry_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
         carry_column(as_global(static_cast<const uint8_t*>(j.masks) + in_off), carry, j.strips, j.W);
 }
 
-template <typename T>
+// CHAIN: the strip also stores its column-chain values at its last row (carry slot s), which the
+// same thread reads back for the next strip.
+template <typename T, bool CHAIN>
 __device__ __forceinline__ void
 integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripRows][kChunk + 1])
 {
@@ -160,6 +168,8 @@ integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripR
             acc = acc + pre[r];
             tile[r][t] = acc;
         }
+        if (CHAIN && s + 1 < j.strips && c0 + t < W)
+            carry[W + c0 + t] = acc; // slot s (rows == kStripRows below the last strip)
         __syncthreads();
         if (c0 + kChunk < W)
             load_chunk(c0 + kChunk);
@@ -226,9 +236,26 @@ integral_strip_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     const int32_t b = int32_t(local / uint32_t(j.strips));
     const int32_t s = int32_t(local - uint64_t(b) * uint32_t(j.strips));
     if (j.dtype == kF32)
-        integral_strip<float>(j, b, s, tile);
+        integral_strip<float, false>(j, b, s, tile);
     else
-        integral_strip<uint8_t>(j, b, s, tile);
+        integral_strip<uint8_t, false>(j, b, s, tile);
+}
+
+// Single pass for batches of many images: one workgroup per image, strips from the top.
+__global__ __launch_bounds__(256) void
+integral_chain_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
+{
+    __shared__ float tile[kStripRows][kChunk + 1];
+    const uint64_t blk = blockIdx.x;
+    const IntegralJob j = jobs[find_job(jobs, njobs, blk, 2)];
+    const int32_t b = int32_t(blk - j.cam_begin);
+    for (int32_t s = 0; s < j.strips; ++s)
+    {
+        if (j.dtype == kF32)
+            integral_strip<float, true>(j, b, s, tile);
+        else
+            integral_strip<uint8_t, true>(j, b, s, tile);
+    }
 }
 
 // Zero row and zero column of the padded layout used by carving (one block per camera).
@@ -279,10 +306,21 @@ integral_job_bytes(size_t njobs)
     return njobs * sizeof(IntegralJob);
 }
 
+// Images from which a batch takes the single-pass form (one workgroup per image keeps every CU busy).
+constexpr uint64_t kIntegralChainImages = 1024;
+
 static void
 integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* carry, bool border, Uploader& up,
                    hipStream_t st)
 {
+    uint64_t images = 0;
+    for (const auto& j : jobs)
+        images += uint64_t(j.B);
+    const bool chain = images >= kIntegralChainImages;
+    if (chain) // largest images first: the longest walks start first
+        std::stable_sort(jobs.begin(), jobs.end(), [](const IntegralJob& a, const IntegralJob& b) {
+            return int64_t(a.H) * a.W > int64_t(b.H) * b.W;
+        });
     uint64_t cols = 0, strips = 0, cams = 0, coff = 0;
     for (auto& j : jobs)
     {
@@ -306,6 +344,12 @@ integral_jobs_impl(std::vector<IntegralJob>& jobs, IntegralJob* djobs, float* ca
     {
         integral_border_kernel<<<unsigned(cams), 256, 0, st>>>(djobs, n);
         THULL_LAUNCH_CHECK();
+    }
+    if (chain)
+    {
+        integral_chain_kernel<<<unsigned(cams), 256, 0, st>>>(djobs, n);
+        THULL_LAUNCH_CHECK();
+        return;
     }
     if (cols > 0)
     {
