@@ -132,31 +132,38 @@ integral_carry_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
         carry_column(as_global(static_cast<const uint8_t*>(j.masks) + in_off), carry, j.strips, j.W);
 }
 
-// CHAIN: the strip also stores its column-chain values at its last row (carry slot s), which the
-// same thread reads back for the next strip.
+// Mask values of strip s, chunk c0 for this thread's column (rows past the image read as 0 and are
+// never stored) and the column chain's value above the strip (carry slot s - 1).
+template <typename T>
+__device__ __forceinline__ void
+integral_load_chunk(const IntegralJob& j, int32_t b, int32_t s, int32_t c0, float (&pre)[kStripRows], float& cin)
+{
+    const int32_t H = j.H, W = j.W;
+    const int32_t r0 = s * kStripRows;
+    const int32_t rows = min(kStripRows, H - r0);
+    const int32_t x = c0 + int32_t(threadIdx.x);
+    const bool in = x < W;
+    const auto src = as_global(static_cast<const T*>(j.masks) + int64_t(b) * H * W + int64_t(r0) * W + x);
+#pragma unroll
+    for (int r = 0; r < kStripRows; ++r)
+        pre[r] = (in && r < rows) ? as_float(src[int64_t(r) * W]) : 0.0f;
+    cin = (in && s > 0) ? j.carry[(int64_t(b) * (j.strips - 1) + (s - 1)) * W + x] : 0.0f;
+}
+
+// One strip; pre / cin hold its chunk 0 (integral_load_chunk) on entry.  CHAIN: the strip also
+// stores its column-chain values at its last row (carry slot s), which the same thread reads back
+// for the next strip, and its last chunk is followed by the reads of the next strip's chunk 0.
 template <typename T, bool CHAIN>
 __device__ __forceinline__ void
-integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripRows][kChunk + 1])
+integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripRows][kChunk + 1],
+               float (&pre)[kStripRows], float& cin)
 {
     const int32_t t = threadIdx.x;
     const int32_t H = j.H, W = j.W, ld = j.ld;
     const int32_t r0 = s * kStripRows;
     const int32_t rows = min(kStripRows, H - r0);
-    const auto src = as_global(static_cast<const T*>(j.masks) + int64_t(b) * H * W + int64_t(r0) * W);
     const auto dst = as_global(j.out + b * j.plane + int64_t(r0) * ld);
-    const auto carry = as_global(j.carry + (int64_t(b) * (j.strips - 1) + (s - 1)) * W);
-    float pre[kStripRows];
-    float cin = 0.0f;
-    // mask values of chunk c0 for this thread's column, rows past the image read as 0 (never stored)
-    const auto load_chunk = [&](int32_t c0) {
-        const int32_t x = c0 + t;
-        const bool in = x < W;
-#pragma unroll
-        for (int r = 0; r < kStripRows; ++r)
-            pre[r] = (in && r < rows) ? as_float(src[int64_t(r) * W + x]) : 0.0f;
-        cin = (in && s > 0) ? carry[x] : 0.0f;
-    };
-    load_chunk(0);
+    const auto carry = as_global(j.carry + (int64_t(b) * (j.strips - 1) + s) * W);
     float racc = 0.0f; // row chain of row t (t < rows)
     for (int32_t c0 = 0; c0 < W; c0 += kChunk)
     {
@@ -169,10 +176,12 @@ integral_strip(const IntegralJob& j, int32_t b, int32_t s, float (&tile)[kStripR
             tile[r][t] = acc;
         }
         if (CHAIN && s + 1 < j.strips && c0 + t < W)
-            carry[W + c0 + t] = acc; // slot s (rows == kStripRows below the last strip)
+            carry[c0 + t] = acc; // slot s (rows == kStripRows below the last strip)
         __syncthreads();
         if (c0 + kChunk < W)
-            load_chunk(c0 + kChunk);
+            integral_load_chunk<T>(j, b, s, c0 + kChunk, pre, cin);
+        else if (CHAIN && s + 1 < j.strips)
+            integral_load_chunk<T>(j, b, s + 1, 0, pre, cin); // carry slot s of chunk 0: stored above
         if (t < rows)
         {
             const int32_t cols = min(kChunk, W - c0);
@@ -235,10 +244,17 @@ integral_strip_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     const uint64_t local = blk - j.strip_begin;
     const int32_t b = int32_t(local / uint32_t(j.strips));
     const int32_t s = int32_t(local - uint64_t(b) * uint32_t(j.strips));
+    float pre[kStripRows], cin;
     if (j.dtype == kF32)
-        integral_strip<float, false>(j, b, s, tile);
+    {
+        integral_load_chunk<float>(j, b, s, 0, pre, cin);
+        integral_strip<float, false>(j, b, s, tile, pre, cin);
+    }
     else
-        integral_strip<uint8_t, false>(j, b, s, tile);
+    {
+        integral_load_chunk<uint8_t>(j, b, s, 0, pre, cin);
+        integral_strip<uint8_t, false>(j, b, s, tile, pre, cin);
+    }
 }
 
 // Single pass for batches of many images: one workgroup per image, strips from the top.
@@ -249,12 +265,18 @@ integral_chain_kernel(const IntegralJob* __restrict__ jobs, int32_t njobs)
     const uint64_t blk = blockIdx.x;
     const IntegralJob j = jobs[find_job(jobs, njobs, blk, 2)];
     const int32_t b = int32_t(blk - j.cam_begin);
-    for (int32_t s = 0; s < j.strips; ++s)
+    float pre[kStripRows], cin;
+    if (j.dtype == kF32)
     {
-        if (j.dtype == kF32)
-            integral_strip<float, true>(j, b, s, tile);
-        else
-            integral_strip<uint8_t, true>(j, b, s, tile);
+        integral_load_chunk<float>(j, b, 0, 0, pre, cin);
+        for (int32_t s = 0; s < j.strips; ++s)
+            integral_strip<float, true>(j, b, s, tile, pre, cin);
+    }
+    else
+    {
+        integral_load_chunk<uint8_t>(j, b, 0, 0, pre, cin);
+        for (int32_t s = 0; s < j.strips; ++s)
+            integral_strip<uint8_t, true>(j, b, s, tile, pre, cin);
     }
 }
 
