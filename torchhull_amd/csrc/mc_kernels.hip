@@ -563,7 +563,11 @@ mc_faces_kernel(McPlan p,
 // Faces of the compact mesh: per complete cell, the vertex index of each of
 // its triangle corners (an edge's rank in the edge numbering), written at the
 // cell's triangle-vertex slots (tbase, a multiple of 3) with flip_faces'
-// column swap.
+// column swap.  The slots of a wavefront's 64 cells are one contiguous range
+// (tbase is an exclusive prefix in vertex order): they are assembled in LDS
+// and written with coalesced stores.
+constexpr int kMcMaxTriVerts = 15; // triangle vertices of one cell (5 triangles)
+
 template <typename Lookup>
 __global__ __launch_bounds__(256) void
 mc_faces_unique_kernel(McPlan p,
@@ -575,30 +579,41 @@ mc_faces_unique_kernel(McPlan p,
                        const uint32_t* __restrict__ scene_vstart,
                        int64_t* __restrict__ faces)
 {
+    __shared__ uint32_t stage[kMcWaves][64 * kMcMaxTriVerts];
     const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
-    if (i >= p.nverts)
-        return;
-    const uint32_t inf = info[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool valid = i < p.nverts;
+    const uint32_t inf = valid ? info[i] : 0u;
+    const uint32_t t0 = valid ? tbase[i] : 0u;
     const uint32_t ci = inf & 0xffu;
     const int count = (inf & kInfoComplete) ? c_cases.count[ci] : 0;
+    // wave range [T0, T1) of triangle-vertex slots; lane 0 holds the smallest valid vertex
+    const uint32_t T0 = __builtin_amdgcn_readfirstlane(t0);
+    uint32_t end = valid ? t0 + uint32_t(count) : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        end = max(end, __shfl_xor(end, d, 64));
     if (count > 0)
     {
-        const uint32_t t0 = tbase[i];
         const VertexRef r = vertex_ref(p, i);
         const McGrid& g = p.grids[r.scene];
         const uint32_t vs = scene_vstart[r.scene];
         // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
         // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
         const uint32_t lows = c_cases.lows[ci];
-        uint32_t pos[8];
-        pos[0] = uint32_t(i);
+        uint32_t vb[8], em[8];
         if constexpr (std::is_same_v<Lookup, BitmapLookup>)
         {
-            // corners 2r and 2r + 1 (x and x + 1) share row r = (dy, dz): one bitmap word and one
-            // prefix per needed row; the +x neighbour of vertex i is i + 1
-            pos[1] = uint32_t(i) + 1u;
+            // corners 2q and 2q + 1 (x and x + 1) of row q = (dy, dz) are consecutive present
+            // vertices: one bitmap word and prefix give the row's rank, and the two corners' vertex
+            // bases and edge masks are read as pairs (element-aligned 8-byte loads)
+            typedef uint32_t pair_t __attribute__((ext_vector_type(2), aligned(4)));
+            typedef __attribute__((address_space(1))) const pair_t global_pair_t;
+            uint32_t pos[4];
+            pos[0] = uint32_t(i);
             uint64_t w[4];
-            uint32_t word[4], b = r.x & 31u;
+            uint32_t word[4];
+            const uint32_t b = r.x & 31u;
 #pragma unroll
             for (int q = 1; q < 4; ++q)
                 if (lows & (3u << (2 * q)))
@@ -610,14 +625,23 @@ mc_faces_unique_kernel(McPlan p,
 #pragma unroll
             for (int q = 1; q < 4; ++q)
                 if (lows & (3u << (2 * q)))
+                    pos[q] = lk.prefix[w[q]] + __popc(word[q] & ((1u << b) - 1u));
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (lows & (3u << (2 * q)))
                 {
-                    // every corner of a complete cell is present
-                    pos[2 * q] = lk.prefix[w[q]] + __popc(word[q] & ((1u << b) - 1u));
-                    pos[2 * q + 1] = b != 31u ? pos[2 * q] + 1u : lk.prefix[w[q] + 1];
+                    const pair_t v2 = *(global_pair_t*)(vbase + pos[q]);
+                    const pair_t e2 = *(global_pair_t*)(emask + pos[q]);
+                    vb[2 * q] = v2.x;
+                    vb[2 * q + 1] = v2.y;
+                    em[2 * q] = e2.x;
+                    em[2 * q + 1] = e2.y;
                 }
         }
         else
         {
+            uint32_t pos[8];
+            pos[0] = uint32_t(i);
             typename Lookup::Probe pr[8];
 #pragma unroll
             for (int o = 1; o < 8; ++o)
@@ -627,35 +651,39 @@ mc_faces_unique_kernel(McPlan p,
             for (int o = 1; o < 8; ++o)
                 if (lows & (1u << o))
                     pos[o] = lk.rank(pr[o]);
-        }
-        uint32_t vb[8], em[8];
 #pragma unroll
-        for (int o = 0; o < 8; ++o)
-            if (lows & (1u << o))
-            {
-                vb[o] = vbase[pos[o]];
-                em[o] = emask[pos[o]];
-            }
-        int64_t* out = faces + uint64_t(t0);
+            for (int o = 0; o < 8; ++o)
+                if (lows & (1u << o))
+                {
+                    vb[o] = vbase[pos[o]];
+                    em[o] = emask[pos[o]];
+                }
+        }
+        uint32_t* out = &stage[wave][t0 - T0];
         for (int k = 0; k < count; ++k)
         {
             const int e = c_cases.edges[ci][k];
             const uint32_t lo = c_edge_low[e];
             const int axis = c_edge_axis[e];
-            uint32_t b = 0u, m = 0u;
+            uint32_t bv = 0u, m = 0u;
 #pragma unroll
             for (int o = 0; o < 8; ++o)
                 if (lo == uint32_t(o))
                 {
-                    b = vb[o];
+                    bv = vb[o];
                     m = em[o];
                 }
             int col = k % 3;
             if (g.flip && col != 0)
                 col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
-            out[k - k % 3 + col] = int64_t(b + __popc(m & ((1u << axis) - 1u)) - vs);
+            out[k - k % 3 + col] = bv + __popc(m & ((1u << axis) - 1u)) - vs;
         }
     }
+    __syncthreads();
+    const uint32_t n = end - T0;
+    int64_t* dst = faces + uint64_t(T0);
+    for (uint32_t q = uint32_t(lane); q < n; q += 64)
+        dst[q] = int64_t(stage[wave][q]);
 }
 
 // ---------------------------------------------------------------------------
