@@ -273,8 +273,6 @@ mc_classify_vertex(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t 
                    uint32_t& ntv)
 {
     const V iso = static_cast<V>(p.iso);
-    const V vg = value_at<V>(p, uint32_t(i));
-    const bool below_g = vg < iso;
     constexpr uint32_t corner[7] = { 1u, 4u, 3u, 2u, 5u, 6u, 7u };
     bool found[7];
     uint32_t pos[7];
@@ -301,17 +299,48 @@ mc_classify_vertex(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t 
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             complete = complete && found[k];
-        pos[0] = uint32_t(i) + 1u; // the +x neighbour of a present vertex follows it
+        // Values are read as pairs of consecutive vertices, the corners x and x + 1 of a row: the
+        // vertex itself and its +x neighbour (which follows it when present), and in the rows
+        // (y + 1, z), (y, z + 1), (y + 1, z + 1) the first present corner x' at rank r of x and, in a
+        // complete cell, x + 1 at r + 1.  A pair starts at a = min(r, nverts - 2) (some neighbour is
+        // present whenever one is read, so nverts >= 2), and the value of r is its first element when
+        // a == r, else its second (r = nverts - 1); the second corner of a pair is only read in a
+        // complete cell, where r + 1 < nverts.
+        typedef V pair_t __attribute__((ext_vector_type(2), aligned(sizeof(V))));
+        typedef __attribute__((address_space(1))) const pair_t global_pair_t;
+        const V* vals = static_cast<const V*>(p.values);
+        const uint32_t last2 = uint32_t(p.nverts) - 2u; // used only when nverts >= 2
+        V vg, vrow[4][2] = {}; // entries of absent corners are never compared
+        if (found[0])
+        {
+            const pair_t v2 = *(global_pair_t*)(vals + i); // i + 1 < nverts
+            vg = v2.x;
+            vrow[0][1] = v2.y;
+        }
+        else
+            vg = vals[i];
+        const auto row_pair = [&](const RowProbe& q, int row) {
+            const uint32_t b = r.x & 31u;
+            const uint32_t r0 = lk.prefix[q.w] + __popc(q.word & ((1u << b) - 1u));
+            const uint32_t a = min(r0, last2);
+            const pair_t v2 = *(global_pair_t*)(vals + a);
+            vrow[row][0] = a == r0 ? v2.x : v2.y;
+            vrow[row][1] = v2.y;
+        };
         if (found[1] || complete)
-            row_ranks(lk, q10, r.x, pos[1], pos[4]);
+            row_pair(q10, 1);
         if (found[2] || complete)
-            row_ranks(lk, q01, r.x, pos[2], pos[3]);
+            row_pair(q01, 2);
         if (complete)
-            row_ranks(lk, q11, r.x, pos[6], pos[5]);
+            row_pair(q11, 3);
+        // case-corner order 1, 4, 3, 2, 5, 6, 7: (x+1,y,z), (x,y+1,z), (x,y,z+1), (x+1,y,z+1),
+        // (x+1,y+1,z), (x+1,y+1,z+1), (x,y+1,z+1)
+        const V cv[7] = { vrow[0][1], vrow[1][0], vrow[2][0], vrow[2][1], vrow[1][1], vrow[3][1], vrow[3][0] };
+        const bool below_g = vg < iso;
         bool below[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k)
-            below[k] = (k < 3 ? found[k] : complete) && value_at<V>(p, pos[k]) < iso;
+            below[k] = (k < 3 ? found[k] : complete) && cv[k] < iso;
         uint32_t sc = 0u;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -330,6 +359,7 @@ mc_classify_vertex(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t 
     }
     else
     {
+        const bool below_g = value_at<V>(p, uint32_t(i)) < iso;
         typename Lookup::Probe pr[7];
         cell_corners(lk, g, i, r, found, pr, inx, iny, inz);
         bool complete = inx && iny && inz; // MC:199
