@@ -470,6 +470,41 @@ mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t
 // LDS and written with coalesced stores.
 constexpr int kMcWaves = 256 / 64;
 
+// Output vertices of the emitted edges m of vertex i into out (3 floats each).
+// UNIFORM as for mc_classify_vertex: keeps the wave-uniform and per-lane copies apart.
+template <typename Lookup, typename V, bool UNIFORM>
+__device__ __forceinline__ void
+mc_edge_vertices(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t i, const VertexRef& r, uint32_t m,
+                 float v1, float* out)
+{
+    const uint32_t p1[3] = { r.x, r.y, r.z };
+    // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
+    // (bitmap policy: the +x endpoint of a present vertex i is vertex i + 1)
+    constexpr bool kBitmap = std::is_same_v<Lookup, BitmapLookup>;
+    typename Lookup::Probe pr[3];
+#pragma unroll
+    for (int d = kBitmap ? 1 : 0; d < 3; ++d)
+        if (m & (1u << d))
+            pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
+    float v2[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (m & (1u << d))
+            v2[d] = float(value_at<V>(p, (kBitmap && d == 0) ? uint32_t(i) + 1u : lk.rank(pr[d])));
+    uint32_t o = 0u;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        if (!(m & (1u << d)))
+            continue;
+        const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
+        float v[3];
+        interpolate_vertex(p.iso, p1, p2, v1, v2[d], v);
+        finish_vertex(g, v, out + 3 * o);
+        ++o;
+    }
+}
+
 template <typename Lookup, typename V>
 __global__ __launch_bounds__(256) void
 mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, const uint32_t* __restrict__ vbase,
@@ -479,8 +514,11 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
     const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool valid = i < p.nverts;
+    // the vertex's own reads are issued together, ahead of the edge test
     const uint32_t m = valid ? emask[i] : 0u;
     const uint32_t out = valid ? vbase[i] : 0u;
+    const VertexRef r = valid ? vertex_ref(p, i) : VertexRef{};
+    const float v1 = valid ? float(value_at<V>(p, uint32_t(i))) : 0.0f; // stored as float (MC:343)
     // wave range [V0, V1) of output vertices; lane 0 holds the smallest valid vertex
     const uint32_t V0 = __builtin_amdgcn_readfirstlane(out);
     uint32_t end = valid ? out + uint32_t(__popc(m & 7u)) : 0u;
@@ -489,35 +527,12 @@ mc_vertices_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ emask, cons
         end = max(end, __shfl_xor(end, d, 64));
     if (m != 0u)
     {
-        const VertexRef r = vertex_ref(p, i);
-        const McGrid& g = p.grids[r.scene];
-        const float v1 = float(value_at<V>(p, uint32_t(i))); // stored as float (MC:343)
-        const uint32_t p1[3] = { r.x, r.y, r.z };
-        // the far endpoints of the emitted edges (present: each edge has a sign change); reads batched
-        // (bitmap policy: the +x endpoint of a present vertex i is vertex i + 1)
-        constexpr bool kBitmap = std::is_same_v<Lookup, BitmapLookup>;
-        typename Lookup::Probe pr[3];
-#pragma unroll
-        for (int d = kBitmap ? 1 : 0; d < 3; ++d)
-            if (m & (1u << d))
-                pr[d] = lk.probe(g, r.x + (d == 0), r.y + (d == 1), r.z + (d == 2));
-        float v2[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (m & (1u << d))
-                v2[d] = float(value_at<V>(p, (kBitmap && d == 0) ? uint32_t(i) + 1u : lk.rank(pr[d])));
-        uint32_t o = out - V0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-        {
-            if (!(m & (1u << d)))
-                continue;
-            const uint32_t p2[3] = { r.x + (d == 0), r.y + (d == 1), r.z + (d == 2) };
-            float v[3];
-            interpolate_vertex(p.iso, p1, p2, v1, v2[d], v);
-            finish_vertex(g, v, &stage[wave][3 * o]);
-            ++o;
-        }
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.scene);
+        float* o = &stage[wave][3 * (out - V0)];
+        if (__all(r.scene == s0)) // one grid for the active lanes: scalar grid parameters
+            mc_edge_vertices<Lookup, V, true>(p, lk, wave_uniform(p.grids[s0]), i, r, m, v1, o);
+        else
+            mc_edge_vertices<Lookup, V, false>(p, lk, p.grids[r.scene], i, r, m, v1, o);
     }
     __syncthreads();
     const uint32_t nf = 3u * (end - V0);
@@ -598,6 +613,95 @@ mc_faces_kernel(McPlan p,
 // and written with coalesced stores.
 constexpr int kMcMaxTriVerts = 15; // triangle vertices of one cell (5 triangles)
 
+// Vertex indices of the triangle corners of complete cell i (case ci, count corners) into out.
+// UNIFORM as for mc_classify_vertex: keeps the wave-uniform and per-lane copies apart.
+template <typename Lookup, bool UNIFORM>
+__device__ __forceinline__ void
+mc_cell_faces(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t i, const VertexRef& r, uint32_t ci,
+              int count, uint32_t vs, const uint32_t* __restrict__ emask, const uint32_t* __restrict__ vbase,
+              uint32_t* out)
+{
+    // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
+    // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
+    const uint32_t lows = c_cases.lows[ci];
+    uint32_t vb[8], em[8];
+    if constexpr (std::is_same_v<Lookup, BitmapLookup>)
+    {
+        // corners 2q and 2q + 1 (x and x + 1) of row q = (dy, dz) are consecutive present
+        // vertices: one bitmap word and prefix give the row's rank, and the two corners' vertex
+        // bases and edge masks are read as pairs (element-aligned 8-byte loads)
+        typedef uint32_t pair_t __attribute__((ext_vector_type(2), aligned(4)));
+        typedef __attribute__((address_space(1))) const pair_t global_pair_t;
+        uint32_t pos[4];
+        pos[0] = uint32_t(i);
+        uint64_t w[4];
+        uint32_t word[4];
+        const uint32_t b = r.x & 31u;
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (lows & (3u << (2 * q)))
+            {
+                w[q] = g.bitmap_offset + (uint64_t(r.z + (q >> 1)) * g.dims[1] + (r.y + (q & 1))) * g.row_words +
+                       (r.x >> 5);
+                word[q] = lk.bitmap[w[q]];
+            }
+#pragma unroll
+        for (int q = 1; q < 4; ++q)
+            if (lows & (3u << (2 * q)))
+                pos[q] = lk.prefix[w[q]] + __popc(word[q] & ((1u << b) - 1u));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (lows & (3u << (2 * q)))
+            {
+                const pair_t v2 = *(global_pair_t*)(vbase + pos[q]);
+                const pair_t e2 = *(global_pair_t*)(emask + pos[q]);
+                vb[2 * q] = v2.x;
+                vb[2 * q + 1] = v2.y;
+                em[2 * q] = e2.x;
+                em[2 * q + 1] = e2.y;
+            }
+    }
+    else
+    {
+        uint32_t pos[8];
+        pos[0] = uint32_t(i);
+        typename Lookup::Probe pr[8];
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
+#pragma unroll
+        for (int o = 1; o < 8; ++o)
+            if (lows & (1u << o))
+                pos[o] = lk.rank(pr[o]);
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (lows & (1u << o))
+            {
+                vb[o] = vbase[pos[o]];
+                em[o] = emask[pos[o]];
+            }
+    }
+    for (int k = 0; k < count; ++k)
+    {
+        const int e = c_cases.edges[ci][k];
+        const uint32_t lo = c_edge_low[e];
+        const int axis = c_edge_axis[e];
+        uint32_t bv = 0u, m = 0u;
+#pragma unroll
+        for (int o = 0; o < 8; ++o)
+            if (lo == uint32_t(o))
+            {
+                bv = vb[o];
+                m = em[o];
+            }
+        int col = k % 3;
+        if (g.flip && col != 0)
+            col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
+        out[k - k % 3 + col] = bv + __popc(m & ((1u << axis) - 1u)) - vs;
+    }
+}
+
 template <typename Lookup>
 __global__ __launch_bounds__(256) void
 mc_faces_unique_kernel(McPlan p,
@@ -613,8 +717,10 @@ mc_faces_unique_kernel(McPlan p,
     const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool valid = i < p.nverts;
+    // the vertex's own reads are issued together, ahead of the case test
     const uint32_t inf = valid ? info[i] : 0u;
     const uint32_t t0 = valid ? tbase[i] : 0u;
+    const VertexRef r = valid ? vertex_ref(p, i) : VertexRef{};
     const uint32_t ci = inf & 0xffu;
     const int count = (inf & kInfoComplete) ? c_cases.count[ci] : 0;
     // wave range [T0, T1) of triangle-vertex slots; lane 0 holds the smallest valid vertex
@@ -625,89 +731,14 @@ mc_faces_unique_kernel(McPlan p,
         end = max(end, __shfl_xor(end, d, 64));
     if (count > 0)
     {
-        const VertexRef r = vertex_ref(p, i);
-        const McGrid& g = p.grids[r.scene];
-        const uint32_t vs = scene_vstart[r.scene];
-        // vertex base and edge mask of every corner that is the lower endpoint of one of the case's
-        // edges (corner 0 is this vertex); all corners of a complete cell are present, reads batched
-        const uint32_t lows = c_cases.lows[ci];
-        uint32_t vb[8], em[8];
-        if constexpr (std::is_same_v<Lookup, BitmapLookup>)
-        {
-            // corners 2q and 2q + 1 (x and x + 1) of row q = (dy, dz) are consecutive present
-            // vertices: one bitmap word and prefix give the row's rank, and the two corners' vertex
-            // bases and edge masks are read as pairs (element-aligned 8-byte loads)
-            typedef uint32_t pair_t __attribute__((ext_vector_type(2), aligned(4)));
-            typedef __attribute__((address_space(1))) const pair_t global_pair_t;
-            uint32_t pos[4];
-            pos[0] = uint32_t(i);
-            uint64_t w[4];
-            uint32_t word[4];
-            const uint32_t b = r.x & 31u;
-#pragma unroll
-            for (int q = 1; q < 4; ++q)
-                if (lows & (3u << (2 * q)))
-                {
-                    w[q] = g.bitmap_offset + (uint64_t(r.z + (q >> 1)) * g.dims[1] + (r.y + (q & 1))) * g.row_words +
-                           (r.x >> 5);
-                    word[q] = lk.bitmap[w[q]];
-                }
-#pragma unroll
-            for (int q = 1; q < 4; ++q)
-                if (lows & (3u << (2 * q)))
-                    pos[q] = lk.prefix[w[q]] + __popc(word[q] & ((1u << b) - 1u));
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (lows & (3u << (2 * q)))
-                {
-                    const pair_t v2 = *(global_pair_t*)(vbase + pos[q]);
-                    const pair_t e2 = *(global_pair_t*)(emask + pos[q]);
-                    vb[2 * q] = v2.x;
-                    vb[2 * q + 1] = v2.y;
-                    em[2 * q] = e2.x;
-                    em[2 * q + 1] = e2.y;
-                }
-        }
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.scene);
+        uint32_t* o = &stage[wave][t0 - T0];
+        if (__all(r.scene == s0)) // one grid for the active lanes: scalar grid parameters
+            mc_cell_faces<Lookup, true>(p, lk, wave_uniform(p.grids[s0]), i, r, ci, count,
+                                        __builtin_amdgcn_readfirstlane(scene_vstart[s0]), emask, vbase, o);
         else
-        {
-            uint32_t pos[8];
-            pos[0] = uint32_t(i);
-            typename Lookup::Probe pr[8];
-#pragma unroll
-            for (int o = 1; o < 8; ++o)
-                if (lows & (1u << o))
-                    pr[o] = lk.probe(g, r.x + (o & 1), r.y + ((o >> 1) & 1), r.z + ((o >> 2) & 1));
-#pragma unroll
-            for (int o = 1; o < 8; ++o)
-                if (lows & (1u << o))
-                    pos[o] = lk.rank(pr[o]);
-#pragma unroll
-            for (int o = 0; o < 8; ++o)
-                if (lows & (1u << o))
-                {
-                    vb[o] = vbase[pos[o]];
-                    em[o] = emask[pos[o]];
-                }
-        }
-        uint32_t* out = &stage[wave][t0 - T0];
-        for (int k = 0; k < count; ++k)
-        {
-            const int e = c_cases.edges[ci][k];
-            const uint32_t lo = c_edge_low[e];
-            const int axis = c_edge_axis[e];
-            uint32_t bv = 0u, m = 0u;
-#pragma unroll
-            for (int o = 0; o < 8; ++o)
-                if (lo == uint32_t(o))
-                {
-                    bv = vb[o];
-                    m = em[o];
-                }
-            int col = k % 3;
-            if (g.flip && col != 0)
-                col = 3 - col; // flip_faces swaps columns 1 and 2 (visual_hull_cuda.cu:951-954)
-            out[k - k % 3 + col] = bv + __popc(m & ((1u << axis) - 1u)) - vs;
-        }
+            mc_cell_faces<Lookup, false>(p, lk, p.grids[r.scene], i, r, ci, count, scene_vstart[r.scene], emask,
+                                         vbase, o);
     }
     __syncthreads();
     const uint32_t n = end - T0;
