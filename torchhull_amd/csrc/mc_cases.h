@@ -65,6 +65,8 @@ struct McCaseTable
     int8_t edges[256][16]; // edge ids per case, -1 terminated
     int8_t count[256];     // number of triangle vertices per case (multiple of 3)
     uint8_t lows[256];     // bit o: cell offset o is the lower endpoint of an edge of the case
+    // per triangle vertex k: (lower endpoint offset << 2) | axis of edge edges[c][k] (0 past count)
+    alignas(16) uint8_t low_axis[256][16];
 };
 
 constexpr int
@@ -89,6 +91,7 @@ mc_build_case_table()
         for (int k = 0; k < n; ++k)
         {
             t.lows[c] = static_cast<uint8_t>(t.lows[c] | (1u << kMcEdgeLow[t.edges[c][k]]));
+            t.low_axis[c][k] = static_cast<uint8_t>((kMcEdgeLow[t.edges[c][k]] << 2) | kMcEdgeAxis[t.edges[c][k]]);
         }
         for (int k = n; k < 16; ++k)
         {

@@ -428,6 +428,32 @@ cell_complete(const McPlan& p, const Lookup& lk, const McGrid& g, const uint32_t
     return (info[pos] & kInfoComplete) != 0u;
 }
 
+// Emitted edges of a vertex with sign-change bits sc (info word inf).
+// UNIFORM as for mc_classify_vertex: keeps the wave-uniform and per-lane copies apart.
+template <typename Lookup, bool UNIFORM>
+__device__ __forceinline__ uint32_t
+mc_vertex_emask(const McPlan& p, const Lookup& lk, const McGrid& g, const uint32_t* __restrict__ info,
+                const VertexRef& r, uint32_t inf, uint32_t sc)
+{
+    uint32_t m = 0u;
+    const int32_t x = int32_t(r.x), y = int32_t(r.y), z = int32_t(r.z);
+    const bool self = (inf & kInfoComplete) != 0u;
+    // cells sharing the edge from (x,y,z) along axis d: origins minus the two other unit vectors
+    if (sc & 1u)
+        if (self || cell_complete(p, lk, g, info, x, y - 1, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
+            cell_complete(p, lk, g, info, x, y - 1, z - 1))
+            m |= 1u;
+    if (sc & 2u)
+        if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
+            cell_complete(p, lk, g, info, x - 1, y, z - 1))
+            m |= 2u;
+    if (sc & 4u)
+        if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y - 1, z) ||
+            cell_complete(p, lk, g, info, x - 1, y - 1, z))
+            m |= 4u;
+    return m;
+}
+
 template <typename Lookup>
 __global__ __launch_bounds__(256) void
 mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t* __restrict__ emask)
@@ -435,28 +461,18 @@ mc_edges_kernel(McPlan p, Lookup lk, const uint32_t* __restrict__ info, uint32_t
     const uint64_t i = uint64_t(xcd_block(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     if (i >= p.nverts)
         return;
+    // the vertex's own reads are issued together, ahead of the sign-change test
     const uint32_t inf = info[i];
+    const VertexRef r = vertex_ref(p, i);
     const uint32_t sc = (inf >> kInfoSignShift) & 7u;
     uint32_t m = 0u;
     if (sc != 0u)
     {
-        const VertexRef r = vertex_ref(p, i);
-        const McGrid& g = p.grids[r.scene];
-        const int32_t x = int32_t(r.x), y = int32_t(r.y), z = int32_t(r.z);
-        const bool self = (inf & kInfoComplete) != 0u;
-        // cells sharing the edge from (x,y,z) along axis d: origins minus the two other unit vectors
-        if (sc & 1u)
-            if (self || cell_complete(p, lk, g, info, x, y - 1, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
-                cell_complete(p, lk, g, info, x, y - 1, z - 1))
-                m |= 1u;
-        if (sc & 2u)
-            if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
-                cell_complete(p, lk, g, info, x - 1, y, z - 1))
-                m |= 2u;
-        if (sc & 4u)
-            if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y - 1, z) ||
-                cell_complete(p, lk, g, info, x - 1, y - 1, z))
-                m |= 4u;
+        const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.scene);
+        if (__all(r.scene == s0)) // one grid for the active lanes: scalar grid parameters
+            m = mc_vertex_emask<Lookup, true>(p, lk, wave_uniform(p.grids[s0]), info, r, inf, sc);
+        else
+            m = mc_vertex_emask<Lookup, false>(p, lk, p.grids[r.scene], info, r, inf, sc);
     }
     emask[i] = m;
 }
@@ -682,11 +698,17 @@ mc_cell_faces(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t i, co
                 em[o] = emask[pos[o]];
             }
     }
-    for (int k = 0; k < count; ++k)
+    // the case's edges as (lower endpoint, axis) codes, all 15 in one 16-byte read
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 la = *reinterpret_cast<const u32x4*>(&c_cases.low_axis[ci][0]);
+    const uint32_t law[4] = { la.x, la.y, la.z, la.w };
+#pragma unroll
+    for (int k = 0; k < kMcMaxTriVerts; ++k)
     {
-        const int e = c_cases.edges[ci][k];
-        const uint32_t lo = c_edge_low[e];
-        const int axis = c_edge_axis[e];
+        if (k >= count)
+            break;
+        const uint32_t code = (law[k >> 2] >> (8 * (k & 3))) & 0xffu;
+        const uint32_t lo = code >> 2, axis = code & 3u;
         uint32_t bv = 0u, m = 0u;
 #pragma unroll
         for (int o = 0; o < 8; ++o)
