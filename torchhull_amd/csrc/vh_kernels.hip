@@ -1082,22 +1082,28 @@ carve_scatter_kernel(const uint64_t* __restrict__ parents,
                      const SceneDev* __restrict__ scenes,
                      uint32_t* __restrict__ cells)
 {
+    // the block's children are one contiguous range of the list: they are assembled in LDS and
+    // written with coalesced stores
     __shared__ uint32_t lds[kCarveThreads / 64];
+    __shared__ uint64_t stage[kCarveThreads * 8];
     const uint32_t t = blockIdx.x * kCarveThreads + threadIdx.x;
     const uint32_t mask = (t < n) ? child_mask[t] : 0u;
+    const uint64_t key = (t < n) ? parents[t] : 0; // read with the mask, ahead of the block sum
     uint32_t total;
-    uint32_t off = block_offsets[blockIdx.x] + block_exclusive_sum(__popc(mask), lds, total);
-    uint64_t key = 0;
+    uint32_t off = block_exclusive_sum(__popc(mask), lds, total);
     if (mask != 0u)
     {
-        key = parents[t];
         const uint32_t s = key_scene(key), x = 2 * key_x(key), y = 2 * key_y(key), z = 2 * key_z(key);
         // VH:694-711: children of a parent in child order (x bit, y bit, z bit)
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (mask & (1u << c))
-                children[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
+                stage[off++] = pack_key(s, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
     }
+    __syncthreads();
+    uint64_t* dst = children + block_offsets[blockIdx.x];
+    for (uint32_t q = threadIdx.x; q < total; q += kCarveThreads)
+        dst[q] = stage[q];
     if (cells != nullptr)
         mark_leaf_children_wave(scenes, child_level, key, mask, cells);
 }
