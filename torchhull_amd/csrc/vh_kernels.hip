@@ -841,7 +841,7 @@ classify_scene(const SceneDev& s, uint64_t key, int32_t level_i, uint32_t& tests
                     : classify_entry<false, false, UNIFORM>(s, key, level_i, tests);
 }
 
-__global__ __launch_bounds__(kCarveThreads) __attribute__((amdgpu_waves_per_eu(4))) void
+__global__ __launch_bounds__(kCarveThreads) void
 carve_classify_kernel(const uint64_t* __restrict__ parents,
                       uint32_t n,
                       int32_t level_i,
