@@ -514,10 +514,12 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, boo
     bool vec = std::is_same_v<Ti, float>;
     for (auto& j : jobs)
         vec = vec && (j.W % 4 == 0) && (reinterpret_cast<uintptr_t>(j.in) % 16 == 0);
+    // the float32 edge pass takes blocks of kEdgeRows rows (fewer registers per thread, more waves)
+    constexpr int kEdgeRows = RB > 32 ? 32 : RB;
     uint64_t pb = 0, eb = 0, fbytes = 0;
     for (auto& j : jobs)
     {
-        j.by = blocks_for(j.H, RB);
+        j.by = blocks_for(j.H, vec ? kEdgeRows : RB);
         j.bx = blocks_for(j.W, vec ? 4 * kBlurBX : kBlurBX);
         j.p_begin = eb; // edge-pass blocks first
         eb += uint64_t(j.bx) * j.by * j.B;
@@ -537,7 +539,7 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, boo
         if (vec)
         {
             if constexpr (std::is_same_v<Ti, float>)
-                blur_edge_flags_f32x4_kernel<RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
+                blur_edge_flags_f32x4_kernel<kEdgeRows><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
         }
         else
             blur_edge_flags_kernel<Ti, RB><<<unsigned(eb), 256, 0, st>>>(djobs, int32_t(jobs.size()), ts);
@@ -545,6 +547,7 @@ run_blur_rb(std::vector<BlurJob>& jobs, BlurJob* djobs, Uploader& up, int k, boo
     }
     for (auto& j : jobs)
     {
+        j.by = blocks_for(j.H, RB);
         j.bx = blocks_for(j.W, kBlurBX);
         j.p_begin = pb;
         pb += uint64_t(j.bx) * j.by * j.B;
