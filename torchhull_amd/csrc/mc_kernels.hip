@@ -629,6 +629,17 @@ mc_faces_kernel(McPlan p,
 // and written with coalesced stores.
 constexpr int kMcMaxTriVerts = 15; // triangle vertices of one cell (5 triangles)
 
+constexpr int
+mc_max_case_count()
+{
+    const McCaseTable t = mc_build_case_table();
+    int m = 0;
+    for (int c = 0; c < 256; ++c)
+        m = t.count[c] > m ? t.count[c] : m;
+    return m;
+}
+static_assert(mc_max_case_count() <= kMcMaxTriVerts, "face staging holds kMcMaxTriVerts slots per cell");
+
 // Vertex indices of the triangle corners of complete cell i (case ci, count corners) into out.
 // UNIFORM as for mc_classify_vertex: keeps the wave-uniform and per-lane copies apart.
 template <typename Lookup, bool UNIFORM>
