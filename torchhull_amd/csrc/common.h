@@ -155,9 +155,8 @@ lerp_ref(float v0, float v1, float t)
 // on the numerator, so both quotients share it.  Other operands take the
 // plain division.
 //
-// A NaN operand gives a NaN quotient on both routes, so the range test may
-// ignore NaN operands (as fmaxf / fminf do): only the NaN-ness of a quotient
-// is used downstream.
+// A NaN operand fails the range test (the magnitude bounds propagate NaN) and
+// takes the plain division.
 __device__ __forceinline__ void
 div2_unchecked(float nx, float ny, float d, float& qx, float& qy)
 {
@@ -170,16 +169,17 @@ div2_unchecked(float nx, float ny, float d, float& qx, float& qy)
     qy = fmaf(fmaf(-d, b, ny), r, b);
 }
 
-// Largest and smallest magnitude of three operands (NaN ignored).
+// Largest and smallest magnitude of three operands; NaN if any of them is NaN
+// (one v_maximum3 / v_minimum3, with no quieting of the operands first).
 __device__ __forceinline__ float
 max_abs3(float a, float b, float c)
 {
-    return fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(c));
+    return __builtin_elementwise_maximum(__builtin_elementwise_maximum(fabsf(a), fabsf(b)), fabsf(c));
 }
 __device__ __forceinline__ float
 min_abs3(float a, float b, float c)
 {
-    return fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));
+    return __builtin_elementwise_minimum(__builtin_elementwise_minimum(fabsf(a), fabsf(b)), fabsf(c));
 }
 
 // True when every operand magnitude between lo and hi lies in [2^-40, 2^40].

@@ -1665,7 +1665,7 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
     {
         // VH:513-518: only cameras whose rounded pixel lies inside with margin 1;
         // 1 <= roundf(p) <= n - 2  <=>  0.5 <= p < n - 1.5 (roundf rounds halves up for p >= 0)
-        f.skip = !(px >= 0.5f && px < float(s.W) - 1.5f && py >= 0.5f && py < float(s.H) - 1.5f);
+        f.skip = !((px >= 0.5f) & (px < float(s.W) - 1.5f) & (py >= 0.5f) & (py < float(s.H) - 1.5f));
     }
     // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
     // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
@@ -1718,10 +1718,14 @@ template <typename T>
 __device__ __forceinline__ float
 field_factor(const FieldSample<T>& f, float pad)
 {
-    const float v00 = f.i00 ? float(f.first0 ? f.p0.x : f.p0.y) : pad;
-    const float v10 = f.i10 ? float(f.first0 ? f.p1.x : f.p1.y) : pad;
-    const float v01 = f.i01 ? float(f.first1 ? f.p0.x : f.p0.y) : pad;
-    const float v11 = f.i11 ? float(f.first1 ? f.p1.x : f.p1.y) : pad;
+    float a0 = float(f.p0.x), b0 = float(f.p0.y), a1 = float(f.p1.x), b1 = float(f.p1.y);
+    // An empty asm statement hides that these values were loaded, so the selects below stay
+    // selects: the compiler otherwise turns a select of a loaded value into branches around it.
+    asm("" : "+v"(a0), "+v"(b0), "+v"(a1), "+v"(b1));
+    const float v00 = f.i00 ? (f.first0 ? a0 : b0) : pad;
+    const float v10 = f.i10 ? (f.first0 ? a1 : b1) : pad;
+    const float v01 = f.i01 ? (f.first1 ? a0 : b0) : pad;
+    const float v11 = f.i11 ? (f.first1 ? a1 : b1) : pad;
     const float l0 = lerp_ref(v00, v10, f.ty);
     const float l1 = lerp_ref(v01, v11, f.ty);
     return lerp_ref(l0, l1, f.tx);
