@@ -145,6 +145,29 @@ def test_sparse_field_matches_oracle(name):
     assert np.array_equal(got[~np.isnan(got)].view(np.uint32), ref.values[~np.isnan(ref.values)].view(np.uint32))
 
 
+@pytest.mark.parametrize("partial", [False, True])
+@pytest.mark.parametrize("convention", ["opengl", "opencv"])
+def test_sparse_field_with_degenerate_camera_matches_oracle(partial, convention):
+    # camera 0 divides by the world x coordinate: grid vertices on the plane x = 0 project to
+    # infinite or NaN pixels, the others far outside or inside the image; its mask is all ones
+    th = _th()
+    sc = _scene("c0_l5")
+    m = _blurred(sc).clone()
+    m[0] = 1.0
+    T = sc.transforms.clone()
+    T[0] = torch.tensor([[0, 1, 0, 0], [0, 0, 1, 0], [1, 0, 0, 0], [1, 0, 0, 0]], dtype=torch.float32)
+    fld = th.sparse_visual_hull_field(m, T.to(DEV), sc.level, sc.cube_corner_bfl, sc.cube_length, partial,
+                                      convention)
+    ref = oracle.sparse_field(m.cpu().numpy(), T.numpy(), sc.level, sc.cube_corner_bfl, sc.cube_length, partial,
+                              convention)
+    R1 = (1 << sc.level) + 1
+    idx = fld.indices().cpu().numpy()
+    assert np.array_equal(idx[3] + idx[2] * R1 + idx[1] * R1 * R1, ref.ids)
+    got = fld.values().cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(ref.values))
+    assert np.array_equal(got[~np.isnan(got)].view(np.uint32), ref.values[~np.isnan(ref.values)].view(np.uint32))
+
+
 @pytest.mark.parametrize("dtype", [torch.uint8, torch.bool, torch.int64, torch.float16, torch.float64])
 def test_visual_hull_mask_dtypes(dtype):
     th = _th()

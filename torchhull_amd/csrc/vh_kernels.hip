@@ -1608,14 +1608,6 @@ launch_div2_check(uint64_t count, uint64_t seed, unsigned long long* mismatches,
 // same float product.
 // ===========================================================================
 
-// float(int64_t(r)) for r = roundf(...) (integral, +-inf or NaN) with the
-// saturating float -> int64 conversion: NaN -> 0, +-inf and |r| >= 2^63 -> +-2^63.
-__device__ __forceinline__ float
-int64_round_trip(float r)
-{
-    return (r != r) ? 0.0f : fminf(fmaxf(r, -9223372036854775808.0f), 9223372036854775808.0f);
-}
-
 // Bilinear sample of one camera's mask at the projection of a world point
 // (VH:470-530, IU:162-214): the four reads are issued by field_sample and
 // combined by field_factor, so that the reads of several cameras are in
@@ -1667,16 +1659,20 @@ field_sample(const SceneDev& s, int32_t b, float wx, float wy, float wz, FieldSa
         // 1 <= roundf(p) <= n - 2  <=>  0.5 <= p < n - 1.5 (roundf rounds halves up for p >= 0)
         f.skip = !((px >= 0.5f) & (px < float(s.W) - 1.5f) & (py >= 0.5f) & (py < float(s.H) - 1.5f));
     }
-    // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) is carried as the float it converts
-    // back to: NaN -> 0, otherwise clamped to [-2^63, 2^63] (INT64_MAX rounds up to 2^63).  The
-    // range tests are exact in float for integral values and H, W < 2^24.
-    const float fy = int64_round_trip(roundf(py + -0.5f));
-    const float fx = int64_round_trip(roundf(px + -0.5f));
-    // rows / columns as unsigned integers: clamping to [-2, 2^30] keeps every value outside [-1, n)
-    // outside, and -1 wraps to 2^32 - 1, so y0 in [0, H) <=> iy < H and y0 + 1 in [0, H) <=> iy + 1 < H
+    // y0 = int64(roundf(py - 0.5)) (saturating, NaN -> 0) enters the factor through the rows it
+    // selects and through ty = py - y0.  It is carried as a float clamped to [-2, 2^30], which
+    // changes neither: a value the clamp moves (or the reference's saturation would) leaves all
+    // four samples outside the image, and such a factor is +0 for full masks (lerps of zeros with
+    // any finite weight; an infinite py gives NaN either way) and never taken for partial masks
+    // (only cameras that see the point inside the image count); a NaN py gives a NaN weight
+    // either way.  The range tests are exact in float for integral values and H, W < 2^24.
+    const float fy = fminf(fmaxf(roundf(py + -0.5f), -2.0f), 1073741824.0f);
+    const float fx = fminf(fmaxf(roundf(px + -0.5f), -2.0f), 1073741824.0f);
+    // rows / columns as unsigned integers: -1 wraps to 2^32 - 1, so y0 in [0, H) <=> iy < H and
+    // y0 + 1 in [0, H) <=> iy + 1 < H
     const uint32_t H = uint32_t(s.H), W = uint32_t(s.W);
-    const uint32_t iy = uint32_t(int32_t(fminf(fmaxf(fy, -2.0f), 1073741824.0f)));
-    const uint32_t ix = uint32_t(int32_t(fminf(fmaxf(fx, -2.0f), 1073741824.0f)));
+    const uint32_t iy = uint32_t(int32_t(fy));
+    const uint32_t ix = uint32_t(int32_t(fx));
     const bool y0in = iy < H, y1in = iy + 1u < H;
     const bool x0in = ix < W, x1in = ix + 1u < W;
     f.i00 = y0in && x0in;
