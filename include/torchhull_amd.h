@@ -54,11 +54,12 @@ int thull_ctx_memory(thull_ctx* ctx, int64_t* peak_bytes, int64_t* reserved_byte
 int thull_ctx_trim(thull_ctx* ctx);
 
 /* Carving launch shape by octree level size, with no counterpart in the reference
- * (its carving launches one thread per node): a level of at most max_parents[0]
- * parents splits each parent's cameras over 64 lanes, up to max_parents[1] over 16,
- * up to max_parents[2] over 4, larger levels use one lane per parent.  Values are
- * non-decreasing; the results do not depend on them.  torchhull_amd.tuning measures
- * and stores the fastest values for a device. */
+ * (its carving launches one thread per node): max_parents holds 4 values; a level of
+ * at most max_parents[0] parents splits each parent's cameras over 64 lanes, up to
+ * max_parents[1] over 16, up to max_parents[2] over 4, up to max_parents[3] over 2,
+ * larger levels use one lane per parent.  Values are non-decreasing; the results do
+ * not depend on them.  torchhull_amd.tuning measures and stores the fastest values
+ * for a device. */
 int thull_ctx_set_carve_tuning(thull_ctx* ctx, const uint32_t* max_parents);
 int thull_ctx_get_carve_tuning(thull_ctx* ctx, uint32_t* max_parents);
 
@@ -130,7 +131,7 @@ int thull_visual_hull_stats(thull_ctx* ctx, int32_t max_levels, int64_t* camera_
 int thull_profile_enable(thull_ctx* ctx, int32_t enabled);
 /* Per carving step i of the last run: classification kernel time (ms, 0 when
  * profiling is off), parent cells, camera tests, resulting child cells and the
- * kernel used (split = 1: one thread per parent; 4/16/64: lanes per parent). */
+ * kernel used (split = 1: one thread per parent; 2/4/16/64: lanes per parent). */
 int thull_profile_carve(thull_ctx* ctx, int32_t max_levels, float* ms, int64_t* parents, int64_t* tests,
                         int64_t* next, int32_t* split);
 

@@ -10,24 +10,26 @@ from torchhull_amd import tuning
 
 
 def test_split_rule_matches_thresholds():
-    t = (100, 1000, 10000)
-    assert [tuning.split_for(n, t) for n in (1, 100, 101, 1000, 1001, 10000, 10001)] == [64, 64, 16, 16, 4, 4, 1]
+    t = (100, 1000, 10000, 100000)
+    ns = (1, 100, 101, 1000, 1001, 10000, 10001, 100000, 100001)
+    assert [tuning.split_for(n, t) for n in ns] == [64, 64, 16, 16, 4, 4, 2, 2, 1]
 
 
 def test_choice_minimises_summed_level_time():
-    # three levels; the fastest G per level is 64, 4 and 1 (a monotone choice exists)
-    sizes = [50, 5000, 500000]
-    ms = {64: [1.0, 9.0, 90.0], 16: [2.0, 5.0, 40.0], 4: [3.0, 2.0, 20.0], 1: [9.0, 8.0, 10.0]}
+    # four levels; the fastest G per level is 64, 4, 2 and 1 (a monotone choice exists)
+    sizes = [50, 5000, 50000, 500000]
+    ms = {64: [1.0, 9.0, 50.0, 90.0], 16: [2.0, 5.0, 30.0, 40.0], 4: [3.0, 2.0, 12.0, 20.0],
+          2: [4.0, 3.0, 11.0, 15.0], 1: [9.0, 8.0, 13.0, 10.0]}
     samples = {g: list(zip(sizes, v)) for g, v in ms.items()}
     chosen, total = tuning.choose_thresholds(samples)
-    assert total == pytest.approx(1.0 + 2.0 + 10.0)
-    assert [tuning.split_for(n, chosen) for n in sizes] == [64, 4, 1]
+    assert total == pytest.approx(1.0 + 2.0 + 11.0 + 10.0)
+    assert [tuning.split_for(n, chosen) for n in sizes] == [64, 4, 2, 1]
 
 
 def test_choice_is_monotone_when_times_are_not():
     # the best per-level G (1, 64) is not monotone in the level size; the rule can only pick monotone shapes
     sizes = [10, 20]
-    ms = {64: [5.0, 1.0], 16: [5.0, 5.0], 4: [5.0, 5.0], 1: [1.0, 5.0]}
+    ms = {64: [5.0, 1.0], 16: [5.0, 5.0], 4: [5.0, 5.0], 2: [5.0, 5.0], 1: [1.0, 5.0]}
     chosen, total = tuning.choose_thresholds({g: list(zip(sizes, v)) for g, v in ms.items()})
     assert total == pytest.approx(6.0)
     g = [tuning.split_for(n, chosen) for n in sizes]
@@ -37,9 +39,9 @@ def test_choice_is_monotone_when_times_are_not():
 def test_load_accepts_only_valid_files(tmp_path):
     p = tmp_path / "t.json"
     assert tuning.load(p) is None
-    p.write_text(json.dumps({"max_parents": [1, 2, 3]}))
-    assert tuning.load(p) == [1, 2, 3]
-    for bad in ([3, 2, 1], [1, 2], [-1, 2, 3], ["a", 2, 3]):
+    p.write_text(json.dumps({"max_parents": [1, 2, 3, 3]}))
+    assert tuning.load(p) == [1, 2, 3, 3]
+    for bad in ([3, 2, 1, 4], [1, 2, 3], [-1, 2, 3, 4], ["a", 2, 3, 4]):
         p.write_text(json.dumps({"max_parents": bad}))
         assert tuning.load(p) is None
     p.write_text("not json")

@@ -874,9 +874,10 @@ thull_ctx_set_carve_tuning(thull_ctx* ctx, const uint32_t* max_parents)
 {
     return guarded([&] {
         THULL_CHECK(ctx != nullptr && max_parents != nullptr, "ctx and max_parents must not be null");
-        THULL_CHECK(max_parents[0] <= max_parents[1] && max_parents[1] <= max_parents[2],
+        THULL_CHECK(max_parents[0] <= max_parents[1] && max_parents[1] <= max_parents[2] &&
+                        max_parents[2] <= max_parents[3],
                     "max_parents must be non-decreasing");
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k)
             ctx->c.tuning.max_parents[k] = max_parents[k];
     });
 }
@@ -886,7 +887,7 @@ thull_ctx_get_carve_tuning(thull_ctx* ctx, uint32_t* max_parents)
 {
     return guarded([&] {
         THULL_CHECK(ctx != nullptr && max_parents != nullptr, "ctx and max_parents must not be null");
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 4; ++k)
             max_parents[k] = ctx->c.tuning.max_parents[k];
     });
 }

@@ -36,12 +36,12 @@ constexpr int kCarveThreads = 256;
 // parent and the number of occupied children per block of kCarveThreads
 // parents.  `tests` accumulates the number of camera tests (early exit).
 // Returns the number of lanes that share one parent's cameras (1: one thread
-// per parent, carve_classify_kernel; 4/16/64: carve_classify_split_kernel<G>).
+// per parent, carve_classify_kernel; 2/4/16/64: carve_classify_split_kernel<G>).
 // Lanes per parent by level size: G = 64 for at most max_parents[0] parents,
 // 16 up to max_parents[1], 4 up to max_parents[2], else 1.
 struct CarveTuning
 {
-    uint32_t max_parents[3] = { 16384u, 131072u, 1048576u };
+    uint32_t max_parents[4] = { 16384u, 131072u, 1048576u, 1048576u }; // G = 64, 16, 4, 2 up to these sizes
 };
 int32_t carve_split_factor(uint32_t n, const CarveTuning& tune);
 int32_t launch_carve_classify(const uint64_t* parents, uint32_t n, int32_t level_i, const SceneDev* scenes,

@@ -1144,6 +1144,13 @@ launch_carve_classify(const uint64_t* parents,
         THULL_LAUNCH_CHECK();
         carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
     }
+    else if (split == 2)
+    {
+        carve_classify_split_kernel<2><<<blocks_for(int64_t(n) * 2, 256), 256, 0, st>>>(parents, n, level_i, scenes,
+                                                                                        child_mask, tests);
+        THULL_LAUNCH_CHECK();
+        carve_count_kernel<<<nb, kCarveThreads, 0, st>>>(child_mask, n, block_counts);
+    }
     else
     {
         carve_classify_kernel<<<nb, kCarveThreads, 0, st>>>(parents, n, level_i, scenes, child_mask, block_counts,
@@ -1163,6 +1170,8 @@ carve_split_factor(uint32_t n, const CarveTuning& tune)
         return 16;
     if (n <= tune.max_parents[2])
         return 4;
+    if (n <= tune.max_parents[3])
+        return 2;
     return 1;
 }
 

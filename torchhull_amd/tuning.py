@@ -1,12 +1,12 @@
 """Autotuner of the carving launch shape (lanes per parent by octree level size).
 
 Carving classifies each level in one launch.  Small levels split a parent's
-cameras over G lanes (64, 16 or 4) to fill the GPU; large ones use one lane per
+cameras over G lanes (64, 16, 4 or 2) to fill the GPU; large ones use one lane per
 parent.  Which G is fastest for a level of n parents depends on the device and
 on the scenes (cameras per scene, early exits), so it is measured:
 `tune_carving` runs a workload once per G with that G forced on every level,
 reads every level's classification time (HIP events, `thull_profile_carve`),
-and picks the three level-size thresholds that minimise the summed time of
+and picks the four level-size thresholds that minimise the summed time of
 all measured levels.  The choice never changes a result, only its speed.
 
 `python -m torchhull_amd.tuning --out torchhull_amd/carve_tuning.json` tunes on
@@ -31,17 +31,18 @@ from . import _lib
 
 DEFAULT_PATH = pathlib.Path(__file__).resolve().parent / "carve_tuning.json"
 _MAX = 0xFFFFFFFF
+_N = 4  # thresholds: G = 64, 16, 4, 2 up to max_parents[0..3], else 1
 # thresholds that force one G on every level
-_FORCE = {64: (_MAX, _MAX, _MAX), 16: (0, _MAX, _MAX), 4: (0, 0, _MAX), 1: (0, 0, 0)}
+_FORCE = {64: (_MAX,) * 4, 16: (0,) + (_MAX,) * 3, 4: (0, 0, _MAX, _MAX), 2: (0, 0, 0, _MAX), 1: (0,) * 4}
 
 
 def set_carve_tuning(ctx: ctypes.c_void_p, max_parents: Sequence[int]) -> None:
-    arr = (ctypes.c_uint32 * 3)(*[int(v) for v in max_parents])
+    arr = (ctypes.c_uint32 * _N)(*[int(v) for v in max_parents])
     _lib.check(_lib.lib().thull_ctx_set_carve_tuning(ctx, arr))
 
 
 def get_carve_tuning(ctx: ctypes.c_void_p) -> list[int]:
-    arr = (ctypes.c_uint32 * 3)()
+    arr = (ctypes.c_uint32 * _N)()
     _lib.check(_lib.lib().thull_ctx_get_carve_tuning(ctx, arr))
     return list(arr)
 
@@ -54,6 +55,8 @@ def split_for(n: int, max_parents: Sequence[int]) -> int:
         return 16
     if n <= max_parents[2]:
         return 4
+    if n <= max_parents[3]:
+        return 2
     return 1
 
 
@@ -82,12 +85,12 @@ def choose_thresholds(samples: dict[int, list[tuple[int, float]]]) -> tuple[list
     levels = samples[1]
     cands = sorted({0, _MAX} | {n for n, _ in levels})
     best, best_t = None, float("inf")
-    for t0, t1, t2 in itertools.combinations_with_replacement(cands, 3):
+    for ts in itertools.combinations_with_replacement(cands, _N):
         t = 0.0
         for i, (n, _) in enumerate(levels):
-            t += samples[split_for(n, (t0, t1, t2))][i][1]
+            t += samples[split_for(n, ts)][i][1]
         if t < best_t:
-            best, best_t = [t0, t1, t2], t
+            best, best_t = list(ts), t
     return best, best_t
 
 
@@ -116,7 +119,7 @@ def load(path: pathlib.Path = DEFAULT_PATH) -> list[int] | None:
     try:
         with open(path) as f:
             v = json.load(f)["max_parents"]
-        if len(v) == 3 and all(isinstance(x, int) and 0 <= x <= _MAX for x in v) and v[0] <= v[1] <= v[2]:
+        if len(v) == _N and all(isinstance(x, int) and 0 <= x <= _MAX for x in v) and v == sorted(v):
             return v
     except (OSError, ValueError, KeyError, TypeError):
         pass
