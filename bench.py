@@ -257,7 +257,7 @@ def main() -> None:
                          "(HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               f"(scenes are independent, so N GPUs would each run their own batch)", flush=True)
         sys.exit(2)
     dev = torch.device("cuda:0")
