@@ -201,6 +201,67 @@ scan_downsweep_kernel(L load, uint64_t n, const uint32_t* block_offsets, uint32_
     }
 }
 
+// Exclusive scan of up to kSmallScan values in one launch of one block: each
+// thread holds a contiguous run of up to kSmallItems inputs in registers (all
+// reads in flight together), the block scans the 64-bit run sums, and each
+// thread writes its run's prefixes.  Writes the total like
+// scan_block_sums_kernel, notification included.
+constexpr int kSmallItems = 32;
+constexpr uint64_t kSmallScan = 1024 * kSmallItems;
+
+template <typename L>
+__global__ __launch_bounds__(1024) void
+scan_small_kernel(L load, uint32_t n, uint32_t* out, unsigned long long* total_out, ScanNotify note = {})
+{
+    __shared__ unsigned long long lds[1024 / 64];
+    const uint32_t per = (n + 1023) / 1024; // <= kSmallItems
+    const uint32_t begin = threadIdx.x * per;
+    uint32_t v[kSmallItems];
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k)
+        v[k] = (uint32_t(k) < per && begin + k < n) ? load(begin + k) : 0u;
+    unsigned long long local = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k)
+        local += v[k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long inc = local;
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        unsigned long long o = __shfl_up(inc, d, 64);
+        if (lane >= d)
+            inc += o;
+    }
+    if (lane == 63)
+        lds[wave] = inc;
+    __syncthreads();
+    unsigned long long off = 0, total = 0;
+    for (int w = 0; w < 1024 / 64; ++w)
+    {
+        if (w < wave)
+            off += lds[w];
+        total += lds[w];
+    }
+    unsigned long long run = off + inc - local;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k)
+    {
+        if (uint32_t(k) < per && begin + k < n)
+            out[begin + k] = uint32_t(run);
+        run += v[k];
+    }
+    if (threadIdx.x == 0)
+    {
+        *total_out = total;
+        if (note.slot != nullptr)
+        {
+            note.slot[0] = total;
+            note.slot[1] = note.extra != nullptr ? *note.extra : 0ull;
+            __hip_atomic_store(note.slot + 2, note.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
 // Exclusive scan of load(i), i in [0, n), into out[0, n).  `block_sums` must
 // hold scan_blocks(n) entries; the 64-bit total lands in *total_dev.
 inline uint32_t
@@ -220,6 +281,12 @@ exclusive_scan(L load, uint64_t n, uint32_t* out, uint32_t* block_sums, unsigned
     {
         THULL_CHECK(note.slot == nullptr, "scan notification of an empty scan");
         THULL_HIP(hipMemsetAsync(total_dev, 0, sizeof(unsigned long long), st));
+        return;
+    }
+    if (n <= kSmallScan)
+    {
+        scan_small_kernel<<<1, 1024, 0, st>>>(load, uint32_t(n), out, total_dev, note);
+        THULL_LAUNCH_CHECK();
         return;
     }
     scan_reduce_kernel<<<nb, kScanThreads, 0, st>>>(load, n, block_sums);
