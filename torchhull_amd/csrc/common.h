@@ -212,17 +212,20 @@ grid_to_world(uint32_t v, float inv_res, float length, float corner)
 
 // ---------------------------------------------------------------------------
 // Padded integral-image layout used by carving: per camera a plane of H + 1
-// rows of pitch integral_pitch(W) floats; element (y, x) is at row y + 1,
-// column x + 4, so every row's data starts 16-byte aligned.  Row 0 and
-// column 3 hold zeros: they are the entries y = -1 and x = -1 that the box
-// sums read at the image border.  Columns 0..2 and those past x = W - 1 are
-// never read.
+// rows of pitch integral_pitch(W) floats (a multiple of 32); element (y, x) is
+// at row y + 1, column x + kIntegralCol0, so every row's data starts on a
+// 128-byte line (the integral passes write whole lines).  Row 0 and column
+// kIntegralCol0 - 1 hold zeros: they are the entries y = -1 and x = -1 that
+// the box sums read at the image border.  The columns before those and past
+// x = W - 1 are never read.
 // ---------------------------------------------------------------------------
+
+constexpr int64_t kIntegralCol0 = 32;
 
 __host__ __device__ __forceinline__ int64_t
 integral_pitch(int64_t W)
 {
-    return ((W + 3) & ~int64_t(3)) + 4;
+    return (W + kIntegralCol0 + 31) & ~int64_t(31);
 }
 
 // ---------------------------------------------------------------------------

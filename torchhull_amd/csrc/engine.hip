@@ -522,7 +522,7 @@ visual_hull_run(Context& ctx, const thull_scene* in, int32_t n, bool unique, hip
         THULL_CHECK(sc.H * sc.W < (int64_t(1) << 31), "H * W must be below 2^31");
         // pixel rounding in float and 32-bit byte offsets into a camera's padded integral plane
         THULL_CHECK(sc.H < (int64_t(1) << 23) && sc.W < (int64_t(1) << 23), "H and W must be below 2^23");
-        THULL_CHECK((sc.H + 1) * integral_pitch(sc.W) < (int64_t(1) << 30), "(H + 1) * (W + 4) must be below 2^30");
+        THULL_CHECK((sc.H + 1) * integral_pitch(sc.W) < (int64_t(1) << 30), "(H + 1) * integral pitch (W + 32, rounded up to 32) must be below 2^30");
         THULL_CHECK(sc.mask_dtype == kF32 || sc.mask_dtype == kU8 || sc.mask_dtype == kBool,
                     "masks must be float32, uint8 or bool at the C ABI");
         SceneDev d{};

@@ -407,7 +407,7 @@ launch_integral_images_padded(const IntegralRequest* req, int32_t n, float* carr
         THULL_CHECK(r.dtype == kF32 || r.dtype == kU8 || r.dtype == kBool,
                     "integral image: unsupported mask element type");
         const int64_t ld = integral_pitch(r.W);
-        jobs.push_back(make_job(r.masks, r.dtype, r.B, r.H, r.W, r.padded + ld + 4, (r.H + 1) * ld, ld));
+        jobs.push_back(make_job(r.masks, r.dtype, r.B, r.H, r.W, r.padded + ld + kIntegralCol0, (r.H + 1) * ld, ld));
     }
     integral_jobs_impl(jobs, static_cast<IntegralJob*>(job_buf), carry, true, up, st);
 }
@@ -767,7 +767,7 @@ parent_camera(const SceneDev& s,
     const uint32_t ld = uint32_t(integral_pitch(s.W));
     const CarveCamera cam = load_camera(s.transforms + 16 * b, GL);
     // entry (y, x), y and x from -1, at I[(y + 1) * ld + x + 1]
-    const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld + 3;
+    const float* I = s.integral + int64_t(b) * (int64_t(s.H) + 1) * ld + (kIntegralCol0 - 1);
 
     // one z-slice of the lattice at a time (only two slices of windows live); the reads of the lower
     // layer are in flight while the top slice is projected and the upper layer's reads are issued
