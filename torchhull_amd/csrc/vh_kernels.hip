@@ -889,15 +889,28 @@ classify_split(const SceneDev& s, uint64_t key, int32_t level_i, int32_t j, uint
     float wx[3], wy[3], wz[3];
     lattice_world(s, key_x(key), key_y(key), key_z(key), level_i, wx, wy, wz);
     const bool last = level_i == s.level - 1;
+    // The group takes its cameras in rounds of G (lane j: camera b0 + j) and shares the children
+    // found empty after every round, so no lane goes on with children another lane has emptied
+    // and the group stops when all children are empty.  A child emptied in a round was live for
+    // every camera before it, so the smallest camera of that round that emptied it is its first.
+    // All lanes of a group have the same parent, so they run the same rounds.
     uint32_t alive = 0xffu;
-    for (int32_t b = j; b < s.B && alive != 0u; b += G)
+    for (int32_t b0 = 0; b0 < s.B; b0 += G)
     {
-        const uint32_t dead = parent_camera<PARTIAL, GL>(s, wx, wy, wz, b, alive, last, refine, inside);
+        const int32_t b = b0 + j;
+        uint32_t dead = 0u;
+        if (b < s.B)
+            dead = parent_camera<PARTIAL, GL>(s, wx, wy, wz, b, alive, last, refine, inside);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (dead & (1u << c))
                 first_empty[c] = uint32_t(b);
         alive &= ~dead;
+#pragma unroll
+        for (int off = G / 2; off >= 1; off >>= 1)
+            alive &= uint32_t(__shfl_xor(int(alive), off, G));
+        if (alive == 0u)
+            break;
     }
 }
 
