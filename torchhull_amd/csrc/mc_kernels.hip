@@ -242,19 +242,14 @@ row_probe(const BitmapLookup& lk, const McGrid& g, uint32_t x, uint32_t y, uint3
     if (!in_row)
         return q;
     q.w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
-    q.word = lk.bitmap[q.w];
     const uint32_t b = x & 31u;
+    // word x + 1 is the next word when x % 32 == 31 (inside the row: x + 1 is in the grid), else
+    // the same word: both reads are issued together, with no branch around the second
+    q.word = lk.bitmap[q.w];
+    q.next = lk.bitmap[q.w + ((inx && b == 31u) ? 1u : 0u)];
     q.p0 = ((q.word >> b) & 1u) != 0u;
     if (inx)
-    {
-        if (b != 31u)
-            q.p1 = ((q.word >> (b + 1u)) & 1u) != 0u;
-        else
-        {
-            q.next = lk.bitmap[q.w + 1];
-            q.p1 = (q.next & 1u) != 0u;
-        }
-    }
+        q.p1 = (b != 31u ? (q.word >> (b + 1u)) & 1u : q.next & 1u) != 0u;
     return q;
 }
 
