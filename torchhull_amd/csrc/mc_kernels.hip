@@ -54,26 +54,26 @@ struct BitmapLookup
     {
         const uint64_t w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
         const uint32_t word = bitmap[w];
+        const uint32_t pre = prefix[w]; // read with the word, not after the presence test
         const uint32_t bit = x & 31u;
         if (((word >> bit) & 1u) == 0u)
             return false;
-        pos = prefix[w] + __popc(word & ((1u << bit) - 1u));
+        pos = pre + __popc(word & ((1u << bit) - 1u));
         return true;
     }
 
-    // Split lookup for batching: probe() issues the bitmap read, present()
-    // and rank() use it, so that the probes of several points are in flight
-    // together.
+    // Split lookup for batching: probe() issues the bitmap and prefix reads,
+    // present() and rank() use them, so that the probes of several points are
+    // in flight together.
     struct Probe
     {
-        uint64_t w;
-        uint32_t word, bit;
+        uint32_t word, pre, bit;
     };
     __device__ __forceinline__ Probe
     probe(const McGrid& g, uint32_t x, uint32_t y, uint32_t z) const
     {
         const uint64_t w = g.bitmap_offset + (uint64_t(z) * g.dims[1] + y) * g.row_words + (x >> 5);
-        return Probe{ w, bitmap[w], x & 31u };
+        return Probe{ bitmap[w], prefix[w], x & 31u };
     }
     __device__ __forceinline__ bool
     present(const Probe& p) const
@@ -83,7 +83,7 @@ struct BitmapLookup
     __device__ __forceinline__ uint32_t
     rank(const Probe& p) const
     {
-        return prefix[p.w] + __popc(p.word & ((1u << p.bit) - 1u));
+        return p.pre + __popc(p.word & ((1u << p.bit) - 1u));
     }
 };
 
