@@ -231,6 +231,7 @@ struct RowProbe
 {
     uint64_t w;     // word of x
     uint32_t word;  // bitmap word of x
+    uint32_t pre;   // prefix of that word (read with it)
     uint32_t next;  // bitmap word of x + 1 when x % 32 == 31 (else unused)
     bool p0, p1;    // x, x + 1 present (x + 1 only when inside the grid)
 };
@@ -247,19 +248,11 @@ row_probe(const BitmapLookup& lk, const McGrid& g, uint32_t x, uint32_t y, uint3
     // the same word: both reads are issued together, with no branch around the second
     q.word = lk.bitmap[q.w];
     q.next = lk.bitmap[q.w + ((inx && b == 31u) ? 1u : 0u)];
+    q.pre = lk.prefix[q.w];
     q.p0 = ((q.word >> b) & 1u) != 0u;
     if (inx)
         q.p1 = (b != 31u ? (q.word >> (b + 1u)) & 1u : q.next & 1u) != 0u;
     return q;
-}
-
-// rank of x (r0) and x + 1 (r1) in a row with at least one of them needed
-__device__ __forceinline__ void
-row_ranks(const BitmapLookup& lk, const RowProbe& q, uint32_t x, uint32_t& r0, uint32_t& r1)
-{
-    const uint32_t b = x & 31u;
-    r0 = lk.prefix[q.w] + __popc(q.word & ((1u << b) - 1u));
-    r1 = b != 31u ? r0 + (q.p0 ? 1u : 0u) : lk.prefix[q.w + 1];
 }
 
 template <typename Lookup, typename V, bool UNIFORM>
@@ -316,7 +309,7 @@ mc_classify_vertex(const McPlan& p, const Lookup& lk, const McGrid& g, uint64_t 
             vg = vals[i];
         const auto row_pair = [&](const RowProbe& q, int row) {
             const uint32_t b = r.x & 31u;
-            const uint32_t r0 = lk.prefix[q.w] + __popc(q.word & ((1u << b) - 1u));
+            const uint32_t r0 = q.pre + __popc(q.word & ((1u << b) - 1u));
             const uint32_t a = min(r0, last2);
             const pair_t v2 = *(global_pair_t*)(vals + a);
             vrow[row][0] = a == r0 ? v2.x : v2.y;
