@@ -403,19 +403,6 @@ mc_classify_kernel(McPlan p, Lookup lk, uint32_t* __restrict__ info, uint32_t* _
 // Pass 2: emitted edges per vertex
 // ---------------------------------------------------------------------------
 
-template <typename Lookup>
-__device__ __forceinline__ bool
-cell_complete(const McPlan& p, const Lookup& lk, const McGrid& g, const uint32_t* info, int32_t x, int32_t y,
-              int32_t z)
-{
-    if (x < 0 || y < 0 || z < 0)
-        return false;
-    uint32_t pos;
-    if (!lk.find(g, uint32_t(x), uint32_t(y), uint32_t(z), pos))
-        return false;
-    return (info[pos] & kInfoComplete) != 0u;
-}
-
 // Emitted edges of a vertex with sign-change bits sc (info word inf).
 // UNIFORM as for mc_classify_vertex: keeps the wave-uniform and per-lane copies apart.
 template <typename Lookup, bool UNIFORM>
@@ -423,22 +410,49 @@ __device__ __forceinline__ uint32_t
 mc_vertex_emask(const McPlan& p, const Lookup& lk, const McGrid& g, const uint32_t* __restrict__ info,
                 const VertexRef& r, uint32_t inf, uint32_t sc)
 {
-    uint32_t m = 0u;
-    const int32_t x = int32_t(r.x), y = int32_t(r.y), z = int32_t(r.z);
-    const bool self = (inf & kInfoComplete) != 0u;
-    // cells sharing the edge from (x,y,z) along axis d: origins minus the two other unit vectors
+    if (inf & kInfoComplete) // the vertex's own cell is complete: every sign-changing edge is emitted
+        return sc;
+    // cells sharing the edge from (x,y,z) along axis d: origins minus the two other unit vectors.
+    // The six cells other than the vertex's own that these edges touch, (dx, dy, dz) offsets
+    // 010, 001, 011, 100, 101, 110 (bit 2: x, 1: y, 0: z), are looked up together: all bitmap reads,
+    // then all info reads, instead of one cell after the other.
+    constexpr uint32_t off[6] = { 2u, 1u, 3u, 4u, 5u, 6u };
+    // cells each axis needs: axis x: 010 001 011, axis y: 100 001 101, axis z: 100 010 110
+    uint32_t need = 0u;
     if (sc & 1u)
-        if (self || cell_complete(p, lk, g, info, x, y - 1, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
-            cell_complete(p, lk, g, info, x, y - 1, z - 1))
-            m |= 1u;
+        need |= 0b000111u;
     if (sc & 2u)
-        if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y, z - 1) ||
-            cell_complete(p, lk, g, info, x - 1, y, z - 1))
-            m |= 2u;
+        need |= 0b011010u;
     if (sc & 4u)
-        if (self || cell_complete(p, lk, g, info, x - 1, y, z) || cell_complete(p, lk, g, info, x, y - 1, z) ||
-            cell_complete(p, lk, g, info, x - 1, y - 1, z))
-            m |= 4u;
+        need |= 0b101001u;
+    typename Lookup::Probe pr[6];
+    bool ok[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        const uint32_t dx = (off[k] >> 2) & 1u, dy = (off[k] >> 1) & 1u, dz = off[k] & 1u;
+        ok[k] = (need & (1u << k)) && r.x >= dx && r.y >= dy && r.z >= dz;
+        if (ok[k])
+            pr[k] = lk.probe(g, r.x - dx, r.y - dy, r.z - dz);
+    }
+    uint32_t inf_k[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        ok[k] = ok[k] && lk.present(pr[k]);
+        inf_k[k] = ok[k] ? info[lk.rank(pr[k])] : 0u;
+    }
+    uint32_t complete = 0u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        complete |= (inf_k[k] & kInfoComplete) ? (1u << k) : 0u;
+    uint32_t m = 0u;
+    if ((sc & 1u) && (complete & 0b000111u))
+        m |= 1u;
+    if ((sc & 2u) && (complete & 0b011010u))
+        m |= 2u;
+    if ((sc & 4u) && (complete & 0b101001u))
+        m |= 4u;
     return m;
 }
 
