@@ -1323,6 +1323,9 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
     const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
     // all lanes take part in the lookup ballot before any diverge
     int32_t lo = find_segment_rising(begin, nscenes, w0);
+    // first word of scene lo + 1, kept in a register (the scan of scene boundaries reads memory only
+    // when a thread's words enter the next scene)
+    uint64_t next_begin = lo + 1 < nscenes ? begin(lo + 1) : ~uint64_t(0);
     int32_t at = -1;           // scene of the position below
     WordPos q{ 0u, 0u, 0u };   // position of the previous word of this thread in scene `at`
     uint32_t step_w = 0u, step_r = 0u, R1 = 0u, rw = 0u;
@@ -1336,8 +1339,11 @@ dilate_cells_kernel(const uint32_t* __restrict__ cells,
             const uint64_t w = w0 + uint64_t(k0 + u) * kScanThreads;
             if (w < nwords)
             {
-                while (lo + 1 < nscenes && begin(lo + 1) <= w)
+                while (w >= next_begin)
+                {
                     ++lo;
+                    next_begin = lo + 1 < nscenes ? begin(lo + 1) : ~uint64_t(0);
+                }
                 const SceneDev& sd = scenes[lo];
                 if (lo != at)
                 {
@@ -1457,6 +1463,11 @@ prefix_extract_kernel(const uint32_t* __restrict__ bitmap,
         run += wave_total[q];
     const auto begin = [&](int32_t i) { return scene_word_begin[i]; };
     int32_t lo = find_segment_rising(begin, nscenes, wbase + lane);
+    // the scene's first word, the next scene's first word and the scene's row geometry, kept in
+    // registers: memory is read only when a thread's words enter another scene
+    uint64_t cur_begin = begin(lo), next_begin = lo + 1 < nscenes ? begin(lo + 1) : ~uint64_t(0);
+    int32_t at = -1;
+    uint32_t rw = 1u, R1 = 1u;
 #pragma unroll
     for (int k = 0; k < kSteps; ++k)
     {
@@ -1468,14 +1479,22 @@ prefix_extract_kernel(const uint32_t* __restrict__ bitmap,
         if (w >= nwords)
             continue;
         prefix[w] = pos;
-        while (lo + 1 < nscenes && begin(lo + 1) <= w)
+        while (w >= next_begin)
+        {
             ++lo;
+            cur_begin = next_begin;
+            next_begin = lo + 1 < nscenes ? begin(lo + 1) : ~uint64_t(0);
+        }
         uint32_t word = v[k];
         if (word == 0u)
             continue;
-        const SceneDev& sd = scenes[lo];
-        const uint32_t rw = sd.row_words, R1 = (1u << sd.level) + 1u;
-        const uint64_t local = w - begin(lo);
+        if (lo != at)
+        {
+            rw = scenes[lo].row_words;
+            R1 = (1u << scenes[lo].level) + 1u;
+            at = lo;
+        }
+        const uint64_t local = w - cur_begin;
         const uint64_t row = div_u64_u32(local, rw);
         const uint32_t xw = uint32_t(local - row * rw);
         const uint64_t zq = div_u64_u32(row, R1);
