@@ -63,19 +63,27 @@ def test_every_carving_launch_shape_gives_the_same_mesh():
     saved = tuning.get_carve_tuning(ctx)
     scenes = synthetic.mixed_scenes(3, seed=77, device=dev, max_level=7)
     blurred = [th.gaussian_blur(sc.masks, sc.blur[0], sc.blur[1], sparse=True, dtype=torch.float32) for sc in scenes]
-    meshes = []
+    import ctypes
+
+    meshes, tests = [], []
     try:
         for force in tuning._FORCE.values():
             tuning.set_carve_tuning(ctx, force)
-            out = []
+            out, cnt = [], []
             for sc, m in zip(scenes, blurred):
                 v, f = th.visual_hull(m, sc.transforms, sc.level, sc.cube_corner_bfl, sc.cube_length,
                                       sc.masks_partial, sc.transforms_convention, unique_verts=True)
                 out.append((v.cpu(), f.cpu()))
+                # camera tests and children per carving step: the canonical early-exit counts
+                ct, ch = (ctypes.c_int64 * 16)(), (ctypes.c_int64 * 16)()
+                _lib.check(_lib.lib().thull_visual_hull_stats(ctx, 16, ct, ch))
+                cnt.append((list(ct), list(ch)))
             meshes.append(out)
+            tests.append(cnt)
     finally:
         tuning.set_carve_tuning(ctx, saved)
     for other in meshes[1:]:
         for (v0, f0), (v1, f1) in zip(meshes[0], other):
             assert torch.equal(f0, f1)
             assert torch.equal(v0.view(torch.int32), v1.view(torch.int32))
+    assert all(t == tests[0] for t in tests[1:])
